@@ -1,0 +1,160 @@
+"""Benchmark: 3D 7-point FP64 Jacobi, 512^3, 1000 sweeps per step, one
+persistent cooperative kernel per step (the single-GPU Jacobi configuration),
+plus CG time-to-1e-8 on the 128^3 Poisson system as side figures.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-cg]
+
+Prints ONE JSON line.  `value` is GLUP/s (interior lattice updates per second)
+over the K timed steps, with the field resident in HBM before timing starts.
+`roofline.achieved` is the algorithmic HBM traffic of one launch
+(16 B per interior lattice update: read a, write a_new) divided by the mean
+HIP-event duration of the launch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "cpu-free-model_amd"))
+
+PEAK_HBM_GBS = 8000.0
+BYTES_PER_LUP_F64 = 16
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--niter", type=int, default=1000)
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--chunks", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(n, target_s):
+    """Oracle (C, OpenMP) 3D 7-point FP64 sweeps on the host cores, on a
+    bounded number of sweeps of the same 512^3 field."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    init = oracle.init3d(n, n, n)
+    t0 = time.perf_counter()
+    oracle.jacobi(init, 1, oracle.BC_DIRICHLET)
+    t1 = time.perf_counter() - t0
+    sweeps = max(1, min(50, int(target_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    oracle.jacobi(init, sweeps, oracle.BC_DIRICHLET)
+    dt = time.perf_counter() - t0
+    lups = (n - 2) ** 3 * sweeps
+    return {"value": lups / dt / 1e9, "unit": "GLUP/s", "cores": cores, "kind": "port",
+            "sample": f"{sweeps} sweeps of the 512^3 FP64 field (C oracle, OpenMP {cores} threads, "
+                      f"{dt:.1f} s incl. one array copy in/out)"}
+
+
+def cg_figures(cfm):
+    out = {}
+    rp, col, val = cfm.poisson3d(128, 128, 128)
+    b = cfm.uniform(0, 0, rp.size - 1)
+    for name, method, mode in (("standard", cfm.CG_STANDARD, cfm.CG_PERSISTENT),
+                               ("pipelined", cfm.CG_PIPELINED, cfm.CG_PERSISTENT),
+                               ("ir_f32", cfm.CG_IR_F32, cfm.CG_PERSISTENT),
+                               ("standard_discrete", cfm.CG_STANDARD, cfm.CG_DISCRETE),
+                               ("pipelined_discrete", cfm.CG_PIPELINED, cfm.CG_DISCRETE)):
+        p = cfm.cg_params(method=method, mode=mode, tol=1e-8, max_iter=10000)
+        with cfm.CgPlan(rp, col, val, p) as plan:
+            plan.set_rhs(b)
+            plan.set_x0(None)
+            plan.solve()  # warm-up
+            best = None
+            for _ in range(3):
+                st, hist, it, t = plan.solve()
+                if best is None or t["kernel_ms"] < best[1]["kernel_ms"]:
+                    best = (it, t, st)
+            it, t, st = best
+            out[name] = {"ms_to_1e-8": round(t["kernel_ms"], 3), "iterations": it,
+                         "outer": t["outer_iterations"], "status": cfm.status_str(st),
+                         "us_per_iter": round(1000 * t["kernel_ms"] / max(it, 1), 2)}
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import cfm
+
+    n, niter = args.n, args.niter
+    dev = cfm.device_info(0)
+    init = cfm.init_field(n, n, n, cfm.INIT_SIN, cfm.F64)
+    p = cfm.jacobi_params(n, n, n, niter=niter, prec=cfm.F64, bc=cfm.BC_DIRICHLET, variant=cfm.PERSISTENT,
+                          tile_rows=args.tile_rows, grid_blocks=args.grid, chunks=args.chunks)
+    plan = cfm.JacobiPlan(p)
+    plan.load(init)
+    del init
+    for _ in range(args.warmup):
+        plan.run()
+    cfm.lib.cfm_device_synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, t = plan.run()
+        kernel_ms.append(t.kernel_ms)
+    cfm.lib.cfm_device_synchronize()
+    wall = time.perf_counter() - t0
+    grid = t.grid_blocks
+    plan.close()
+
+    interior = (n - 2) ** 3
+    lups = interior * niter * args.steps
+    value = lups / wall / 1e9
+    mean_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    achieved = BYTES_PER_LUP_F64 * interior * niter / mean_kernel_s / 1e9
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "jacobi3d_f64_512_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    line = {
+        "metric": "3D Jacobi GLUP/s + HBM %peak; CG time-to-1e-8 (learned/batched vs plain)",
+        "value": round(value, 2),
+        "unit": "GLUP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (sin*sin Dirichlet x faces, zero interior)",
+        "config": {"workload": "jacobi3d_7pt_persistent", "nx": n, "ny": n, "nz": n, "sweeps_per_step": niter,
+                   "bc": "dirichlet", "grid_blocks": grid, "device": dev["name"], "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "jacobi3d_kernel<double,4,true>",
+                     "mean_launch_ms": round(1000 * mean_kernel_s, 3)},
+    }
+    if not args.no_cg:
+        line["cg_128"] = cg_figures(cfm)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,328 @@
+"""ctypes binding of libcfm.so, the MI355X-native CPU-free solver library.
+
+Mirrors the reference's per-version entry points for the single-GPU path:
+
+* ``jacobi(...)``   -- the stencil versions' ``init(argc, argv)``
+  (Stencil/jacobi2D/src/main.cu:25-68, Stencil/jacobi3D/src/main.cu:31-72):
+  allocate, initialise both arrays, run ``niter`` sweeps, copy back.
+* ``cg(...)``       -- the CG versions' ``init(rowptr_d, col_d, val_d, n, nnz,
+  zero_indexed, iter_max, x_final, ...)`` (CG/include/common.h:122-133).
+
+Every call goes through the C ABI declared in include/cfm.h; there is no CPU
+fallback: if the shared library is missing, importing the module raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfm.so")
+
+OK, EINVAL, ENOMEM, EDEVICE, ETIMEOUT, ENOTCONV, EBREAKDOWN, ENOTSUP = range(8)
+F64, F32 = 0, 1
+BC_PERIODIC, BC_DIRICHLET = 0, 1
+LAUNCH_PER_ITER, PERSISTENT, NO_COMPUTE = 0, 1, 2
+INIT_SIN, INIT_REF_QUIRK = 0, 1
+CG_STANDARD, CG_PIPELINED, CG_IR_F32 = 0, 1, 2
+CG_PERSISTENT, CG_DISCRETE = 0, 1
+
+# Every symbol include/cfm.h declares.
+EXPORTS = (
+    "cfm_version", "cfm_status_str", "cfm_device_count", "cfm_device_info", "cfm_device_synchronize",
+    "cfm_jacobi_plan_create", "cfm_jacobi_plan_load", "cfm_jacobi_plan_run", "cfm_jacobi_plan_download",
+    "cfm_jacobi_plan_field", "cfm_jacobi_plan_destroy", "cfm_jacobi_run", "cfm_jacobi_init_field",
+    "cfm_cg_plan_create", "cfm_cg_plan_set_rhs", "cfm_cg_plan_set_x0", "cfm_cg_plan_solve",
+    "cfm_cg_plan_get_x", "cfm_cg_plan_destroy", "cfm_cg_solve", "cfm_gen_poisson3d", "cfm_gen_uniform",
+)
+
+
+class CfmError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        super().__init__(f"{where}: {status_str(status)} ({status})")
+
+
+class JacobiParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "nx", "ny", "nz", "niter", "norm_every", "prec", "bc", "variant", "tile_rows", "grid_blocks",
+        "chunks", "spin_limit_us")]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("grid_blocks", ctypes.c_int), ("block_threads", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CgParams(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("mode", ctypes.c_int), ("tol", ctypes.c_double),
+                ("max_iter", ctypes.c_int), ("restart_quirk", ctypes.c_int), ("max_outer", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("grid_blocks", ctypes.c_int),
+                ("spin_limit_us", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C cpu-free-model_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.cfm_version.restype = ctypes.c_char_p
+    L.cfm_status_str.restype = ctypes.c_char_p
+    L.cfm_status_str.argtypes = [ctypes.c_int]
+    L.cfm_device_count.argtypes = [_ip]
+    L.cfm_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _ip,
+                                  ctypes.POINTER(ctypes.c_size_t)]
+    L.cfm_jacobi_plan_create.argtypes = [ctypes.POINTER(JacobiParams), ctypes.POINTER(_vp)]
+    L.cfm_jacobi_plan_load.argtypes = [_vp, _vp]
+    L.cfm_jacobi_plan_run.argtypes = [_vp, _dp, ctypes.c_int, _ip, ctypes.POINTER(Timing)]
+    L.cfm_jacobi_plan_download.argtypes = [_vp, _vp]
+    L.cfm_jacobi_plan_field.argtypes = [_vp, ctypes.POINTER(_vp)]
+    L.cfm_jacobi_plan_destroy.argtypes = [_vp]
+    L.cfm_jacobi_plan_destroy.restype = None
+    L.cfm_jacobi_run.argtypes = [ctypes.POINTER(JacobiParams), _vp, _vp, _dp, ctypes.c_int, _ip,
+                                 ctypes.POINTER(Timing)]
+    L.cfm_jacobi_init_field.argtypes = [ctypes.c_int] * 5 + [_vp]
+    L.cfm_cg_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(CgParams), ctypes.POINTER(_vp)]
+    L.cfm_cg_plan_set_rhs.argtypes = [_vp, _dp]
+    L.cfm_cg_plan_set_x0.argtypes = [_vp, _dp]
+    L.cfm_cg_plan_solve.argtypes = [_vp, _dp, ctypes.c_int, _ip, ctypes.POINTER(Timing)]
+    L.cfm_cg_plan_get_x.argtypes = [_vp, _dp]
+    L.cfm_cg_plan_destroy.argtypes = [_vp]
+    L.cfm_cg_plan_destroy.restype = None
+    L.cfm_cg_solve.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, _dp, _dp,
+                               ctypes.POINTER(CgParams), _dp, ctypes.c_int, _ip, ctypes.POINTER(Timing)]
+    L.cfm_gen_poisson3d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp]
+    L.cfm_gen_poisson3d.restype = ctypes.c_int64
+    L.cfm_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _dp, ctypes.c_int64]
+    return L
+
+
+lib = _load()
+
+
+def status_str(s):
+    return lib.cfm_status_str(int(s)).decode()
+
+
+def version():
+    return lib.cfm_version().decode()
+
+
+def _check(st, where):
+    if st != OK:
+        raise CfmError(st, where)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    st = lib.cfm_device_count(ctypes.byref(n))
+    return n.value if st == OK else 0
+
+
+def device_info(dev=0):
+    name = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int(0)
+    mem = ctypes.c_size_t(0)
+    _check(lib.cfm_device_info(dev, name, 256, ctypes.byref(cus), ctypes.byref(mem)), "cfm_device_info")
+    return {"name": name.value.decode(), "compute_units": cus.value, "mem_bytes": mem.value}
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------ Jacobi --
+
+def init_field(nx, ny, nz=1, kind=INIT_SIN, prec=F64):
+    shape = (ny, nx) if nz == 1 else (nz, ny, nx)
+    a = np.empty(shape, dtype=np.float64 if prec == F64 else np.float32)
+    _check(lib.cfm_jacobi_init_field(nx, ny, nz, kind, prec, _ptr(a)), "cfm_jacobi_init_field")
+    return a
+
+
+def jacobi_params(nx, ny, nz=1, niter=1000, norm_every=0, prec=F64, bc=BC_DIRICHLET,
+                  variant=PERSISTENT, tile_rows=0, grid_blocks=0, chunks=0, spin_limit_us=0):
+    return JacobiParams(nx, ny, nz, niter, norm_every, prec, bc, variant, tile_rows, grid_blocks, chunks,
+                        spin_limit_us)
+
+
+class JacobiPlan:
+    """Device-resident Jacobi state (two working arrays) for repeated runs."""
+
+    def __init__(self, params):
+        self.params = params
+        self.dtype = np.float64 if params.prec == F64 else np.float32
+        self.shape = (params.ny, params.nx) if params.nz == 1 else (params.nz, params.ny, params.nx)
+        h = _vp()
+        _check(lib.cfm_jacobi_plan_create(ctypes.byref(params), ctypes.byref(h)), "cfm_jacobi_plan_create")
+        self._h = h
+
+    def load(self, field):
+        field = np.ascontiguousarray(field, dtype=self.dtype)
+        if field.shape != self.shape:
+            raise ValueError(f"field shape {field.shape} != {self.shape}")
+        _check(lib.cfm_jacobi_plan_load(self._h, _ptr(field)), "cfm_jacobi_plan_load")
+
+    def run(self):
+        p = self.params
+        cap = (p.niter + p.norm_every - 1) // p.norm_every if p.norm_every > 0 else 0
+        norms = np.zeros(max(cap, 1))
+        cnt = ctypes.c_int(0)
+        t = Timing()
+        _check(lib.cfm_jacobi_plan_run(self._h, norms.ctypes.data_as(_dp), cap, ctypes.byref(cnt),
+                                       ctypes.byref(t)), "cfm_jacobi_plan_run")
+        return norms[: cnt.value], t
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib.cfm_jacobi_plan_download(self._h, _ptr(out)), "cfm_jacobi_plan_download")
+        return out
+
+    def close(self):
+        if self._h:
+            lib.cfm_jacobi_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def jacobi(init, niter, bc=BC_DIRICHLET, norm_every=0, variant=PERSISTENT, **tuning):
+    """Run niter sweeps on a (ny, nx) or (nz, ny, nx) float64/float32 field.
+    Returns (field, norms, timing_dict)."""
+    init = np.ascontiguousarray(init)
+    prec = F64 if init.dtype == np.float64 else F32
+    if init.dtype not in (np.float64, np.float32):
+        raise TypeError("float64 or float32 field expected")
+    if init.ndim == 2:
+        ny, nx = init.shape
+        nz = 1
+    else:
+        nz, ny, nx = init.shape
+    p = jacobi_params(nx, ny, nz, niter, norm_every, prec, bc, variant, **tuning)
+    with JacobiPlan(p) as plan:
+        plan.load(init)
+        norms, t = plan.run()
+        out = plan.download()
+    return out, norms, t.as_dict()
+
+
+# ---------------------------------------------------------------------- CG --
+
+def poisson3d(nx, ny, nz):
+    nnz = lib.cfm_gen_poisson3d(nx, ny, nz, None, None, None)
+    if nnz < 0:
+        raise ValueError("grid too large")
+    n = nx * ny * nz
+    rowptr = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    lib.cfm_gen_poisson3d(nx, ny, nz, rowptr.ctypes.data_as(_ip), col.ctypes.data_as(_ip),
+                          val.ctypes.data_as(_dp))
+    return rowptr, col, val
+
+
+def uniform(seed, stream_id, count):
+    out = np.empty(count, dtype=np.float64)
+    _check(lib.cfm_gen_uniform(seed, stream_id, out.ctypes.data_as(_dp), count), "cfm_gen_uniform")
+    return out
+
+
+def cg_params(method=CG_STANDARD, mode=CG_PERSISTENT, tol=1e-8, max_iter=10000, restart_quirk=1,
+              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0):
+    return CgParams(method, mode, tol, max_iter, restart_quirk, max_outer, inner_tol, grid_blocks,
+                    spin_limit_us)
+
+
+class CgPlan:
+    """Device-resident CSR (stored as SELL-64) plus CG work vectors."""
+
+    def __init__(self, rowptr, col, val, params, index_base=0):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = self.rowptr.size - 1
+        self.params = params
+        h = _vp()
+        _check(lib.cfm_cg_plan_create(self.n, self.col.size, _ptr(self.rowptr), _ptr(self.col), _ptr(self.val),
+                                      index_base, 0, ctypes.byref(params), ctypes.byref(h)),
+               "cfm_cg_plan_create")
+        self._h = h
+
+    def set_rhs(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib.cfm_cg_plan_set_rhs(self._h, b.ctypes.data_as(_dp)), "cfm_cg_plan_set_rhs")
+
+    def set_x0(self, x0=None):
+        if x0 is None:
+            _check(lib.cfm_cg_plan_set_x0(self._h, None), "cfm_cg_plan_set_x0")
+        else:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            _check(lib.cfm_cg_plan_set_x0(self._h, x0.ctypes.data_as(_dp)), "cfm_cg_plan_set_x0")
+
+    def solve(self, hist_cap=None):
+        """Returns (status, hist, iters, timing). ENOTCONV/EBREAKDOWN are statuses, not exceptions."""
+        p = self.params
+        if hist_cap is None:
+            hist_cap = (p.max_outer if p.method == CG_IR_F32 and p.max_outer > 0 else
+                        50 if p.method == CG_IR_F32 else p.max_iter) + 1
+        hist = np.zeros(max(hist_cap, 1))
+        it = ctypes.c_int(0)
+        t = Timing()
+        st = lib.cfm_cg_plan_solve(self._h, hist.ctypes.data_as(_dp), hist_cap, ctypes.byref(it),
+                                   ctypes.byref(t))
+        if st not in (OK, ENOTCONV, EBREAKDOWN):
+            raise CfmError(st, "cfm_cg_plan_solve")
+        nh = (t.outer_iterations if p.method == CG_IR_F32 else it.value) + 1
+        return st, hist[: min(nh, hist_cap)], it.value, t.as_dict()
+
+    def x(self):
+        out = np.empty(self.n)
+        _check(lib.cfm_cg_plan_get_x(self._h, out.ctypes.data_as(_dp)), "cfm_cg_plan_get_x")
+        return out
+
+    def close(self):
+        if self._h:
+            lib.cfm_cg_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cg(rowptr, col, val, b, x0=None, index_base=0, **kw):
+    """One-shot solve. Returns (status, x, hist, iters, timing)."""
+    p = cg_params(**kw)
+    with CgPlan(rowptr, col, val, p, index_base) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(x0)
+        st, hist, it, t = plan.solve()
+        x = plan.x()
+    return st, x, hist, it, t

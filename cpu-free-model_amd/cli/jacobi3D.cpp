@@ -1,0 +1,4 @@
+// jacobi3D: 3D 7-point Jacobi driver with the reference's CLI.
+#include "jacobi_cli.hpp"
+
+int main(int argc, char *argv[]) { return cli::run_jacobi(argc, argv, 3); }

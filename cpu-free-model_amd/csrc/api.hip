@@ -1,0 +1,136 @@
+// Library-level entry points: version, status strings, device queries and
+// the host-side generators (initial fields, Poisson CSR, seeded uniforms).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "cfm_common.hpp"
+
+extern "C" const char *cfm_version(void) { return "cfm 0.1.0 (gfx950, HIP)"; }
+
+extern "C" const char *cfm_status_str(int s) {
+    switch (s) {
+        case CFM_OK: return "ok";
+        case CFM_EINVAL: return "invalid argument";
+        case CFM_ENOMEM: return "out of memory";
+        case CFM_EDEVICE: return "device error";
+        case CFM_ETIMEOUT: return "device barrier timed out";
+        case CFM_ENOTCONV: return "not converged";
+        case CFM_EBREAKDOWN: return "CG breakdown (matrix not SPD)";
+        case CFM_ENOTSUP: return "not supported";
+        default: return "unknown status";
+    }
+}
+
+extern "C" cfm_status cfm_device_count(int *count) {
+    if (!count) return CFM_EINVAL;
+    *count = 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return CFM_EDEVICE;
+    *count = n;
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_device_info(int dev, char *name, int name_cap, int *compute_units,
+                                      size_t *mem_bytes) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || dev < 0 || dev >= n) return CFM_EDEVICE;
+    hipDeviceProp_t prop;
+    CFM_HIP(hipGetDeviceProperties(&prop, dev));
+    if (name && name_cap > 0) std::snprintf(name, name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (mem_bytes) *mem_bytes = prop.totalGlobalMem;
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_device_synchronize(void) {
+    CFM_HIP(hipDeviceSynchronize());
+    return CFM_OK;
+}
+
+// ------------------------------------------------------- initial fields ---
+template <typename T>
+static void init_field(int nx, int ny, int nz, int kind, T *a) {
+    const size_t plane = (size_t)nx * ny;
+    std::memset(a, 0, sizeof(T) * plane * (nz >= 3 ? nz : 1));
+    const double pi = 2.0 * std::asin(1.0);
+    if (nz < 3) {
+        // Stencil/jacobi2D/src/common.cu:15-26 (offset 0)
+        for (int iy = 0; iy < ny; ++iy) {
+            const double y0 = std::sin(2.0 * pi * iy / (ny - 1));
+            a[(size_t)iy * nx] = (T)y0;
+            a[(size_t)iy * nx + nx - 1] = (T)y0;
+        }
+        return;
+    }
+    if (kind == CFM_INIT_REF_QUIRK) {
+        // Stencil/jacobi3D/src/common.cu:18-28: the x=0 face keeps the value of
+        // the last ix iteration, (iz - iy) - (nx - 1).
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                a[iz * plane + (size_t)iy * nx] = (T)(((double)iz - (double)iy) - (double)(nx - 1));
+        return;
+    }
+    for (int iz = 0; iz < nz; ++iz) {
+        const double sz = std::sin(2.0 * pi * iz / (nz - 1));
+        for (int iy = 0; iy < ny; ++iy) {
+            const double v = std::sin(2.0 * pi * iy / (ny - 1)) * sz;
+            a[iz * plane + (size_t)iy * nx] = (T)v;
+            a[iz * plane + (size_t)iy * nx + nx - 1] = (T)v;
+        }
+    }
+}
+
+extern "C" cfm_status cfm_jacobi_init_field(int nx, int ny, int nz, int kind, int prec, void *out) {
+    if (!out || nx < 3 || ny < 3 || nz < 1 || nz == 2) return CFM_EINVAL;
+    if (kind != CFM_INIT_SIN && kind != CFM_INIT_REF_QUIRK) return CFM_EINVAL;
+    if (prec == CFM_F64)
+        init_field<double>(nx, ny, nz, kind, static_cast<double *>(out));
+    else if (prec == CFM_F32)
+        init_field<float>(nx, ny, nz, kind, static_cast<float *>(out));
+    else
+        return CFM_EINVAL;
+    return CFM_OK;
+}
+
+// ----------------------------------------------------------- generators ---
+extern "C" int64_t cfm_gen_poisson3d(int nx, int ny, int nz, int *rowptr, int *col, double *val) {
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    const long long n = (long long)nx * ny * nz;
+    if (n * 7 > 0x7fffffffLL) return -1;
+    const long long plane = (long long)nx * ny;
+    long long k = 0, row = 0;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix, ++row) {
+                if (rowptr) rowptr[row] = (int)k;
+                // ascending column order: z-1, y-1, x-1, diagonal, x+1, y+1, z+1
+                const long long cols[7] = {row - plane, row - nx, row - 1, row, row + 1, row + nx, row + plane};
+                const bool ok[7] = {iz > 0, iy > 0, ix > 0, true, ix + 1 < nx, iy + 1 < ny, iz + 1 < nz};
+                for (int j = 0; j < 7; ++j) {
+                    if (!ok[j]) continue;
+                    if (col) col[k] = (int)cols[j];
+                    if (val) val[k] = j == 3 ? 6.0 : -1.0;
+                    ++k;
+                }
+            }
+    if (rowptr) rowptr[row] = (int)k;
+    return k;
+}
+
+extern "C" cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count) {
+    if (!out || count < 0) return CFM_EINVAL;
+    // SplitMix64 (Steele, Lea & Flood, OOPSLA 2014); stream state
+    // seed ^ (golden-ratio increment * (stream_id + 1)); 53-bit uniforms.
+    uint64_t s = seed ^ (0x9E3779B97F4A7C15ULL * (stream_id + 1));
+    for (int64_t i = 0; i < count; ++i) {
+        s += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        out[i] = (double)(z >> 11) * 0x1.0p-53;
+    }
+    return CFM_OK;
+}

@@ -1,0 +1,983 @@
+// CPU-free Conjugate Gradient for gfx950: Standard, Pipelined and an FP32
+// inner CG inside an FP64 iterative-refinement loop, each as one cooperative
+// persistent kernel; plus a host-driven ("discrete") schedule of the same
+// device phases for comparison.
+//
+// Reference algorithms restated:
+//   Standard   CG/src/common.cu:399-538 and CG/src/single-stream/standard.cu:158-299
+//   Pipelined  CG/src/common.cu:609-780 and CG/src/single-stream/pipelined.cu:176-318
+//   SpMV       CG/src/common.cu:150-168 (per-row sum in stored column order)
+//   dot        CG/src/common.cu:357-389 (double accumulation)
+//
+// MI355X design:
+//   * The CSR matrix is stored as SELL-64: slices of 64 consecutive rows, the
+//     j-th entry of every row of a slice contiguous, so the 64 lanes of a
+//     wavefront read col/val with one coalesced load each.  Per-row sums run
+//     in the stored column order, so results equal the CSR formula exactly.
+//     Padding entries carry col = -1 and are skipped.
+//   * Rows are grouped in chunks of 1024 (16 slices); workgroup b owns chunks
+//     b, b+G, ... in every phase, so the vectors a workgroup updates stay in
+//     its XCD's L2 between phases.
+//   * Dot products are deterministic and independent of the grid size: each
+//     chunk produces one partial with a fixed tree, and every workgroup
+//     reduces all partials in the same fixed order after the barrier (no
+//     atomics, no second barrier).  The CPU oracle replicates the tree, so
+//     residual histories agree bit for bit.
+//   * Pipelined CG needs one device-wide barrier per iteration: the two dot
+//     products, q = A w and all six vector updates are fused, with w and
+//     the partials double-buffered.  Standard CG needs three.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cfm_common.hpp"
+
+namespace cfm {
+
+constexpr int kChunk = 1024;
+constexpr int kThreads = 256;
+constexpr int kSlicesPerChunk = kChunk / 64;
+
+struct Sell {
+    const int *slice_ptr;  // [nslices + 1], entry offsets (multiples of 64)
+    const int *col;        // [slice_ptr[nslices]], -1 = padding
+    const void *val;       // double or float
+};
+
+struct CgArgs {
+    int n;
+    int nchunks;
+    Sell A64;  // FP64 values
+    Sell A32;  // FP32 values (IR only)
+    const double *b;
+    const double *x0;
+    double *x;
+    double *v[6];        // work vectors (meaning depends on the method)
+    float *f[4];         // FP32 work vectors (IR)
+    double *part;        // partial sums, [8][nchunks]
+    double *hist;
+    int hist_cap;
+    int max_iter;
+    double tol;
+    int quirk;
+    int max_outer;
+    double inner_tol;
+    int *info;           // [0] status, [1] iterations, [2] outer iterations
+    GridSync *sync;
+};
+
+// ------------------------------------------------------------ SpMV row ----
+template <typename VT, typename XT>
+__device__ __forceinline__ XT sell_row(const Sell &S, const XT *__restrict__ x, int slice, int lane) {
+    const int beg = S.slice_ptr[slice];
+    const int width = (S.slice_ptr[slice + 1] - beg) >> 6;
+    const int *cp = S.col + beg + lane;
+    const VT *vp = static_cast<const VT *>(S.val) + beg + lane;
+    XT acc = XT(0);
+    for (int j = 0; j < width; j += 8) {
+        int c[8];
+        VT v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool ok = j + u < width;
+            c[u] = ok ? cp[(j + u) * 64] : -1;
+            v[u] = ok ? vp[(j + u) * 64] : VT(0);
+        }
+        XT xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : XT(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c[u] >= 0) acc = fma((XT)v[u], xv[u], acc);
+    }
+    return acc;
+}
+
+// Reduce nchunks partials (fixed order) -> same value in every thread.
+__device__ __forceinline__ double reduce_partials(const double *part, int nchunks, double *lds4) {
+    double a = 0.0;
+    for (int j = threadIdx.x; j < nchunks; j += kThreads) a = a + part[j];
+    return block256_sum(a, lds4);
+}
+
+__device__ __forceinline__ bool is_finite(double v) { return v - v == 0.0; }
+
+// Iterate the rows this thread owns: chunk c, rows c*1024 + i*256 + t.
+#define CFM_FOR_MY_ROWS(...)                                                               \
+    for (int c_ = blockIdx.x; c_ < A.nchunks; c_ += gridDim.x) {                            \
+        _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                  \
+            const int slice = c_ * kSlicesPerChunk + i_ * 4 + (threadIdx.x >> 6);          \
+            const int row = c_ * kChunk + i_ * kThreads + threadIdx.x;                      \
+            const int lane = threadIdx.x & 63;                                              \
+            (void)slice;                                                                    \
+            (void)lane;                                                                     \
+            if (row < A.n) {                                                                \
+                __VA_ARGS__                                                                 \
+            }                                                                               \
+        }                                                                                   \
+        CHUNK_END                                                                           \
+    }
+
+// =============================================================== phases ===
+// Standard CG.  v[0]=r, v[1]=p, v[2]=s.  part: [0]=gamma, [1]=delta, [2]=bb.
+struct StdPhases {
+    // r = b - A x0 ; p = r ; x = x0 ; partials <r,r>, <b,b>
+    static __device__ void init(const CgArgs &A, double *lds4) {
+        double *r = A.v[0], *p = A.v[1];
+        double ag = 0.0, ab = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sg = block256_sum(ag, lds4);                                           \
+        const double sb = block256_sum(ab, lds4);                                           \
+        if (threadIdx.x == 0) {                                                             \
+            A.part[0 * A.nchunks + c_] = sg;                                                \
+            A.part[2 * A.nchunks + c_] = sb;                                                \
+        }                                                                                   \
+        ag = 0.0;                                                                           \
+        ab = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const double ax = sell_row<double, double>(A.A64, A.x0, slice, lane);
+            const double bi = A.b[row];
+            const double ri = fma(-1.0, ax, bi);
+            r[row] = ri;
+            p[row] = ri;
+            A.x[row] = A.x0[row];
+            ag = fma(ri, ri, ag);
+            ab = fma(bi, bi, ab);
+        })
+#undef CHUNK_END
+    }
+    // s = A p ; partial <p,s>
+    static __device__ void spmv_dot(const CgArgs &A, double *lds4) {
+        const double *p = A.v[1];
+        double *s = A.v[2];
+        double ad = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sd = block256_sum(ad, lds4);                                           \
+        if (threadIdx.x == 0) A.part[1 * A.nchunks + c_] = sd;                              \
+        ad = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const double si = sell_row<double, double>(A.A64, p, slice, lane);
+            s[row] = si;
+            ad = fma(p[row], si, ad);
+        })
+#undef CHUNK_END
+    }
+    // x += alpha p ; r -= alpha s ; partial <r,r>
+    static __device__ void update_xr(const CgArgs &A, double alpha, double *lds4) {
+        double *r = A.v[0];
+        const double *p = A.v[1], *s = A.v[2];
+        double ag = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sg = block256_sum(ag, lds4);                                           \
+        if (threadIdx.x == 0) A.part[0 * A.nchunks + c_] = sg;                              \
+        ag = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            A.x[row] = fma(alpha, p[row], A.x[row]);
+            const double ri = fma(-alpha, s[row], r[row]);
+            r[row] = ri;
+            ag = fma(ri, ri, ag);
+        })
+#undef CHUNK_END
+    }
+    // p = r + beta p
+    static __device__ void update_p(const CgArgs &A, double beta) {
+        const double *r = A.v[0];
+        double *p = A.v[1];
+#define CHUNK_END
+        CFM_FOR_MY_ROWS({ p[row] = fma(beta, p[row], r[row]); })
+#undef CHUNK_END
+    }
+};
+
+// Pipelined CG.  v[0]=r, v[1]=z, v[2]=s, v[3]=p, v[4]=w0, v[5]=w1.
+// part: [0,1] = delta (buffers 0/1), [2,3] = gamma (buffers 0/1), [4] = bb.
+struct PipePhases {
+    // r = b - A x0 ; x = x0 ; z = s = p = 0 ; partial <b,b>
+    static __device__ void init_r(const CgArgs &A, double *lds4) {
+        double ab = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sb = block256_sum(ab, lds4);                                           \
+        if (threadIdx.x == 0) A.part[4 * A.nchunks + c_] = sb;                              \
+        ab = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const double ax = sell_row<double, double>(A.A64, A.x0, slice, lane);
+            const double bi = A.b[row];
+            A.v[0][row] = fma(-1.0, ax, bi);
+            A.x[row] = A.x0[row];
+            A.v[1][row] = 0.0;
+            A.v[2][row] = 0.0;
+            A.v[3][row] = 0.0;
+            ab = fma(bi, bi, ab);
+        })
+#undef CHUNK_END
+    }
+    // w0 = A r ; partials <r,r>, <r,w0> into buffer 0
+    static __device__ void init_w(const CgArgs &A, double *lds4) {
+        double ad = 0.0, ag = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sd = block256_sum(ad, lds4);                                           \
+        const double sg = block256_sum(ag, lds4);                                           \
+        if (threadIdx.x == 0) {                                                             \
+            A.part[0 * A.nchunks + c_] = sd;                                                \
+            A.part[2 * A.nchunks + c_] = sg;                                                \
+        }                                                                                   \
+        ad = 0.0;                                                                           \
+        ag = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const double wi = sell_row<double, double>(A.A64, A.v[0], slice, lane);
+            A.v[4][row] = wi;
+            const double ri = A.v[0][row];
+            ad = fma(ri, ri, ad);
+            ag = fma(ri, wi, ag);
+        })
+#undef CHUNK_END
+    }
+    // q = A w_cur ; z,s,p,x,r,w updates ; partials for the next iteration
+    static __device__ void iterate(const CgArgs &A, int buf, double alpha, double beta,
+                                   double *lds4) {
+        const double *wc = A.v[4 + buf];
+        double *wn = A.v[5 - buf];
+        double *r = A.v[0], *z = A.v[1], *s = A.v[2], *p = A.v[3];
+        const int nb = buf ^ 1;
+        double ad = 0.0, ag = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sd = block256_sum(ad, lds4);                                           \
+        const double sg = block256_sum(ag, lds4);                                           \
+        if (threadIdx.x == 0) {                                                             \
+            A.part[(0 + nb) * A.nchunks + c_] = sd;                                         \
+            A.part[(2 + nb) * A.nchunks + c_] = sg;                                         \
+        }                                                                                   \
+        ad = 0.0;                                                                           \
+        ag = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const double qi = sell_row<double, double>(A.A64, wc, slice, lane);
+            const double wi = wc[row];
+            const double zi = fma(beta, z[row], qi);
+            const double si = fma(beta, s[row], wi);
+            const double pi = fma(beta, p[row], r[row]);
+            A.x[row] = fma(alpha, pi, A.x[row]);
+            const double ri = fma(-alpha, si, r[row]);
+            const double wn_i = fma(-alpha, zi, wi);
+            z[row] = zi;
+            s[row] = si;
+            p[row] = pi;
+            r[row] = ri;
+            wn[row] = wn_i;
+            ad = fma(ri, ri, ad);
+            ag = fma(ri, wn_i, ag);
+        })
+#undef CHUNK_END
+    }
+};
+
+// FP32 inner CG inside FP64 refinement.  v[0] = r64.  f[0]=rf, f[1]=p, f[2]=s, f[3]=d.
+// part: [0]=rr (outer), [1]=g (inner <r,r>), [2]=delta, [3]=bb.
+struct IrPhases {
+    // r64 = b - A x ; rf = (float) r64 ; p = rf ; d = 0 ; partials rr, g (and bb when first)
+    static __device__ void outer(const CgArgs &A, bool first, double *lds4) {
+        double arr = 0.0, ag = 0.0, ab = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double s1 = block256_sum(arr, lds4);                                          \
+        const double s2 = block256_sum(ag, lds4);                                           \
+        const double s3 = first ? block256_sum(ab, lds4) : 0.0;                             \
+        if (threadIdx.x == 0) {                                                             \
+            A.part[0 * A.nchunks + c_] = s1;                                                \
+            A.part[1 * A.nchunks + c_] = s2;                                                \
+            if (first) A.part[3 * A.nchunks + c_] = s3;                                     \
+        }                                                                                   \
+        arr = 0.0;                                                                          \
+        ag = 0.0;                                                                           \
+        ab = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const double *xs = first ? A.x0 : A.x;
+            const double ax = sell_row<double, double>(A.A64, xs, slice, lane);
+            const double bi = A.b[row];
+            const double ri = fma(-1.0, ax, bi);
+            if (first) A.x[row] = A.x0[row];
+            A.v[0][row] = ri;
+            const float rf = (float)ri;
+            A.f[0][row] = rf;
+            A.f[1][row] = rf;
+            A.f[3][row] = 0.0f;
+            arr = fma(ri, ri, arr);
+            ag = fma((double)rf, (double)rf, ag);
+            ab = fma(bi, bi, ab);
+        })
+#undef CHUNK_END
+    }
+    static __device__ void spmv_dot(const CgArgs &A, double *lds4) {
+        const float *p = A.f[1];
+        float *s = A.f[2];
+        double ad = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sd = block256_sum(ad, lds4);                                           \
+        if (threadIdx.x == 0) A.part[2 * A.nchunks + c_] = sd;                              \
+        ad = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            const float si = sell_row<float, float>(A.A32, p, slice, lane);
+            s[row] = si;
+            ad = fma((double)p[row], (double)si, ad);
+        })
+#undef CHUNK_END
+    }
+    static __device__ void update_dr(const CgArgs &A, float af, double *lds4) {
+        float *rf = A.f[0], *d = A.f[3];
+        const float *p = A.f[1], *s = A.f[2];
+        double ag = 0.0;
+#define CHUNK_END                                                                           \
+    {                                                                                       \
+        const double sg = block256_sum(ag, lds4);                                           \
+        if (threadIdx.x == 0) A.part[1 * A.nchunks + c_] = sg;                              \
+        ag = 0.0;                                                                           \
+    }
+        CFM_FOR_MY_ROWS({
+            d[row] = fmaf(af, p[row], d[row]);
+            const float ri = fmaf(-af, s[row], rf[row]);
+            rf[row] = ri;
+            ag = fma((double)ri, (double)ri, ag);
+        })
+#undef CHUNK_END
+    }
+    static __device__ void update_p(const CgArgs &A, float bf) {
+        const float *rf = A.f[0];
+        float *p = A.f[1];
+#define CHUNK_END
+        CFM_FOR_MY_ROWS({ p[row] = fmaf(bf, p[row], rf[row]); })
+#undef CHUNK_END
+    }
+    static __device__ void update_x(const CgArgs &A) {
+        const float *d = A.f[3];
+#define CHUNK_END
+        CFM_FOR_MY_ROWS({ A.x[row] = A.x[row] + (double)d[row]; })
+#undef CHUNK_END
+    }
+};
+
+// ======================================================= persistent kernels
+__device__ __forceinline__ void write_info(const CgArgs &A, int status, int iters, int outer) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.info[0] = status;
+        A.info[1] = iters;
+        A.info[2] = outer;
+    }
+}
+__device__ __forceinline__ void write_hist(const CgArgs &A, int k, double v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && A.hist && k < A.hist_cap) A.hist[k] = v;
+}
+
+__global__ void __launch_bounds__(256) cg_standard_kernel(CgArgs A) {
+    __shared__ double lds4[4];
+    unsigned long long epoch = 0;
+    StdPhases::init(A, lds4);
+    if (!grid_barrier(A.sync, epoch)) return;
+    double gamma = reduce_partials(A.part + 0 * A.nchunks, A.nchunks, lds4);
+    const double bb = reduce_partials(A.part + 2 * A.nchunks, A.nchunks, lds4);
+    const double thr = A.tol > 0.0 ? A.tol * A.tol * bb : 0.0;
+    write_hist(A, 0, gamma);
+    int k = 0, status = CFM_OK;
+    bool converged = gamma <= thr;
+    while (!converged && k < A.max_iter) {
+        StdPhases::spmv_dot(A, lds4);
+        if (!grid_barrier(A.sync, epoch)) return;
+        const double delta = reduce_partials(A.part + 1 * A.nchunks, A.nchunks, lds4);
+        if (!(delta > 0.0) || !is_finite(delta)) {
+            status = CFM_EBREAKDOWN;
+            break;
+        }
+        const double alpha = gamma / delta;
+        StdPhases::update_xr(A, alpha, lds4);
+        if (!grid_barrier(A.sync, epoch)) return;
+        const double gamma_new = reduce_partials(A.part + 0 * A.nchunks, A.nchunks, lds4);
+        ++k;
+        write_hist(A, k, gamma_new);
+        if (gamma_new <= thr) {
+            converged = true;
+            break;
+        }
+        const double beta = gamma_new / gamma;
+        StdPhases::update_p(A, beta);
+        if (!grid_barrier(A.sync, epoch)) return;
+        gamma = gamma_new;
+    }
+    if (status == CFM_OK && !converged && A.tol > 0.0) status = CFM_ENOTCONV;
+    write_info(A, status, k, 0);
+}
+
+__global__ void __launch_bounds__(256) cg_pipelined_kernel(CgArgs A) {
+    __shared__ double lds4[4];
+    unsigned long long epoch = 0;
+    PipePhases::init_r(A, lds4);
+    if (!grid_barrier(A.sync, epoch)) return;
+    PipePhases::init_w(A, lds4);
+    if (!grid_barrier(A.sync, epoch)) return;
+    const double bb = reduce_partials(A.part + 4 * A.nchunks, A.nchunks, lds4);
+    const double thr = A.tol > 0.0 ? A.tol * A.tol * bb : 0.0;
+    double alpha = 0.0, beta = 0.0, delta_prev = 0.0;
+    int k = 0, status = CFM_OK;
+    bool converged = false;
+    for (;;) {
+        const int buf = k & 1;
+        const double delta = reduce_partials(A.part + (0 + buf) * A.nchunks, A.nchunks, lds4);
+        const double gam = reduce_partials(A.part + (2 + buf) * A.nchunks, A.nchunks, lds4);
+        write_hist(A, k, delta);
+        if (delta <= thr) {
+            converged = true;
+            break;
+        }
+        if (k >= A.max_iter) break;
+        double denom;
+        if (A.quirk ? (k > 1) : (k > 0)) {
+            beta = delta / delta_prev;
+            denom = gam - (beta / alpha) * delta;
+        } else {
+            beta = 0.0;
+            denom = gam;
+        }
+        if (!(denom > 0.0) || !is_finite(denom)) {
+            status = CFM_EBREAKDOWN;
+            break;
+        }
+        alpha = delta / denom;
+        PipePhases::iterate(A, buf, alpha, beta, lds4);
+        if (!grid_barrier(A.sync, epoch)) return;
+        delta_prev = delta;
+        ++k;
+    }
+    if (status == CFM_OK && !converged && A.tol > 0.0) status = CFM_ENOTCONV;
+    write_info(A, status, k, 0);
+}
+
+__global__ void __launch_bounds__(256) cg_ir_kernel(CgArgs A) {
+    __shared__ double lds4[4];
+    unsigned long long epoch = 0;
+    double thr = 0.0;
+    int outer = 0, inner_total = 0, status = CFM_ENOTCONV;
+    for (;;) {
+        const bool first = outer == 0;
+        IrPhases::outer(A, first, lds4);
+        if (!grid_barrier(A.sync, epoch)) return;
+        const double rr = reduce_partials(A.part + 0 * A.nchunks, A.nchunks, lds4);
+        if (first) {
+            const double bb = reduce_partials(A.part + 3 * A.nchunks, A.nchunks, lds4);
+            thr = A.tol * A.tol * bb;
+        }
+        write_hist(A, outer, rr);
+        if (rr <= thr) {
+            status = CFM_OK;
+            break;
+        }
+        if (outer == A.max_outer) break;
+        double g = reduce_partials(A.part + 1 * A.nchunks, A.nchunks, lds4);
+        const double thr_in = A.inner_tol * A.inner_tol * g;
+        int j = 0;
+        bool brk = false;
+        while (j < A.max_iter) {
+            IrPhases::spmv_dot(A, lds4);
+            if (!grid_barrier(A.sync, epoch)) return;
+            const double delta = reduce_partials(A.part + 2 * A.nchunks, A.nchunks, lds4);
+            if (!(delta > 0.0) || !is_finite(delta)) {
+                brk = true;
+                break;
+            }
+            const float af = (float)(g / delta);
+            IrPhases::update_dr(A, af, lds4);
+            if (!grid_barrier(A.sync, epoch)) return;
+            const double gn = reduce_partials(A.part + 1 * A.nchunks, A.nchunks, lds4);
+            ++j;
+            if (gn <= thr_in) break;
+            const float bf = (float)(gn / g);
+            IrPhases::update_p(A, bf);
+            if (!grid_barrier(A.sync, epoch)) return;
+            g = gn;
+        }
+        inner_total += j;
+        IrPhases::update_x(A);
+        if (!grid_barrier(A.sync, epoch)) return;
+        ++outer;
+        if (brk && j == 0) {
+            status = CFM_EBREAKDOWN;
+            break;
+        }
+    }
+    write_info(A, status, inner_total, outer);
+}
+
+// ===================================================== discrete kernels ===
+// One launch per phase; scalars are reduced by a one-workgroup kernel and
+// read back by the host, which forms alpha and beta itself.
+__global__ void __launch_bounds__(256) k_std_init(CgArgs A) {
+    __shared__ double lds4[4];
+    StdPhases::init(A, lds4);
+}
+__global__ void __launch_bounds__(256) k_std_spmv_dot(CgArgs A) {
+    __shared__ double lds4[4];
+    StdPhases::spmv_dot(A, lds4);
+}
+__global__ void __launch_bounds__(256) k_std_update_xr(CgArgs A, double alpha) {
+    __shared__ double lds4[4];
+    StdPhases::update_xr(A, alpha, lds4);
+}
+__global__ void __launch_bounds__(256) k_std_update_p(CgArgs A, double beta) {
+    StdPhases::update_p(A, beta);
+}
+__global__ void __launch_bounds__(256) k_pipe_init_r(CgArgs A) {
+    __shared__ double lds4[4];
+    PipePhases::init_r(A, lds4);
+}
+__global__ void __launch_bounds__(256) k_pipe_init_w(CgArgs A) {
+    __shared__ double lds4[4];
+    PipePhases::init_w(A, lds4);
+}
+__global__ void __launch_bounds__(256) k_pipe_iterate(CgArgs A, int buf, double alpha, double beta) {
+    __shared__ double lds4[4];
+    PipePhases::iterate(A, buf, alpha, beta, lds4);
+}
+// out[i] = reduce(part + idx[i]*nchunks), i < count
+__global__ void __launch_bounds__(256) k_reduce(CgArgs A, int i0, int i1, double *out) {
+    __shared__ double lds4[4];
+    const double a = reduce_partials(A.part + i0 * A.nchunks, A.nchunks, lds4);
+    double b = 0.0;
+    if (i1 >= 0) b = reduce_partials(A.part + i1 * A.nchunks, A.nchunks, lds4);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+}  // namespace cfm
+
+using namespace cfm;
+
+// ==================================================================== host
+struct cfm_cg_plan {
+    cfm_cg_params p{};
+    DeviceProps dev{};
+    int n = 0, nnz = 0, nslices = 0, nchunks = 0;
+    int *slice_ptr = nullptr, *scol = nullptr;
+    double *sval = nullptr;
+    float *sval32 = nullptr;
+    double *b = nullptr, *x0 = nullptr, *x = nullptr;
+    double *v[6] = {};
+    float *f[4] = {};
+    double *part = nullptr, *hist = nullptr, *scal = nullptr, *scal_h = nullptr;
+    int *info = nullptr;
+    int hist_cap = 0;
+    GridSync *sync = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int grid = 0;
+    unsigned long long spin_ticks = 0;
+};
+
+// Build SELL-64 on the host from CSR (rows padded to the slice width, col -1).
+static cfm_status build_sell(int n, const int *rowptr, const int *col, const double *val, int base,
+                             std::vector<int> &sp, std::vector<int> &sc, std::vector<double> &sv) {
+    const int nslices = (n + 63) / 64;
+    sp.assign(nslices + 1, 0);
+    long long total = 0;
+    for (int s = 0; s < nslices; ++s) {
+        int w = 0;
+        for (int r = s * 64; r < std::min(n, s * 64 + 64); ++r) {
+            const int len = rowptr[r + 1] - rowptr[r];
+            if (len < 0) return CFM_EINVAL;
+            w = std::max(w, len);
+        }
+        sp[s] = (int)total;
+        total += 64LL * w;
+        if (total > 0x7fffffffLL) return CFM_EINVAL;
+    }
+    sp[nslices] = (int)total;
+    sc.assign((size_t)total, -1);
+    sv.assign((size_t)total, 0.0);
+    for (int r = 0; r < n; ++r) {
+        const int s = r / 64, lane = r % 64;
+        const int beg = rowptr[r] - base;
+        const int len = rowptr[r + 1] - rowptr[r];
+        for (int j = 0; j < len; ++j) {
+            const int c = col[beg + j] - base;
+            if (c < 0 || c >= n) return CFM_EINVAL;
+            sc[(size_t)sp[s] + (size_t)j * 64 + lane] = c;
+            sv[(size_t)sp[s] + (size_t)j * 64 + lane] = val[beg + j];
+        }
+    }
+    return CFM_OK;
+}
+
+using CgKernelFn = void (*)(CgArgs);
+
+static CgKernelFn persistent_kernel(int method) {
+    switch (method) {
+        case CFM_CG_STANDARD: return cg_standard_kernel;
+        case CFM_CG_PIPELINED: return cg_pipelined_kernel;
+        case CFM_CG_IR_F32: return cg_ir_kernel;
+        default: return nullptr;
+    }
+}
+
+extern "C" void cfm_cg_plan_destroy(cfm_cg_plan *pl) {
+    if (!pl) return;
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    void *ptrs[] = {pl->slice_ptr, pl->scol, pl->sval, pl->sval32, pl->b, pl->x0, pl->x,
+                    pl->part, pl->hist, pl->scal, pl->info, pl->sync};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    for (double *q : pl->v)
+        if (q) (void)hipFree(q);
+    for (float *q : pl->f)
+        if (q) (void)hipFree(q);
+    if (pl->scal_h) (void)hipHostFree(pl->scal_h);
+    if (pl->ev0) (void)hipEventDestroy(pl->ev0);
+    if (pl->ev1) (void)hipEventDestroy(pl->ev1);
+    if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    delete pl;
+}
+
+extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, const int *col,
+                                         const double *val, int index_base, int csr_on_device,
+                                         const cfm_cg_params *p, cfm_cg_plan **out) {
+    if (!out) return CFM_EINVAL;
+    *out = nullptr;
+    if (!p || n < 1 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) return CFM_EINVAL;
+    if (index_base != 0 && index_base != 1) return CFM_EINVAL;
+    if (p->method < CFM_CG_STANDARD || p->method > CFM_CG_IR_F32) return CFM_EINVAL;
+    if (p->mode != CFM_CG_PERSISTENT && p->mode != CFM_CG_DISCRETE) return CFM_EINVAL;
+    if (p->mode == CFM_CG_DISCRETE && p->method == CFM_CG_IR_F32) return CFM_ENOTSUP;
+    if (p->max_iter < 0 || n > 0x7fffffff - 2048) return CFM_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_EDEVICE;
+
+    // host copies of the CSR (device CSR is borrowed and read back once)
+    std::vector<int> h_rowptr, h_col;
+    std::vector<double> h_val;
+    const int *rp = rowptr, *cp = col;
+    const double *vp = val;
+    if (csr_on_device) {
+        h_rowptr.resize((size_t)n + 1);
+        h_col.resize((size_t)nnz);
+        h_val.resize((size_t)nnz);
+        if (hipMemcpy(h_rowptr.data(), rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost) != hipSuccess ||
+            (nnz > 0 && hipMemcpy(h_col.data(), col, sizeof(int) * nnz, hipMemcpyDeviceToHost) != hipSuccess) ||
+            (nnz > 0 && hipMemcpy(h_val.data(), val, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess))
+            return CFM_EDEVICE;
+        rp = h_rowptr.data();
+        cp = h_col.data();
+        vp = h_val.data();
+    }
+    if (rp[0] != index_base || rp[n] - index_base != nnz) return CFM_EINVAL;
+    std::vector<int> sp, sc;
+    std::vector<double> sv;
+    cfm_status st = build_sell(n, rp, cp, vp, index_base, sp, sc, sv);
+    if (st != CFM_OK) return st;
+
+    auto *pl = new (std::nothrow) cfm_cg_plan();
+    if (!pl) return CFM_ENOMEM;
+    pl->p = *p;
+    if (pl->p.max_outer <= 0) pl->p.max_outer = 50;
+    if (pl->p.inner_tol <= 0.0) pl->p.inner_tol = 1e-4;
+    pl->n = n;
+    pl->nnz = nnz;
+    pl->nslices = (int)sp.size() - 1;
+    pl->nchunks = (n + kChunk - 1) / kChunk;
+    auto fail = [&](cfm_status s) {
+        cfm_cg_plan_destroy(pl);
+        return s;
+    };
+    st = current_device_props(&pl->dev);
+    if (st != CFM_OK) return fail(st);
+    // slices are read per chunk: pad the slice table to whole chunks
+    const int nslices_pad = pl->nchunks * kSlicesPerChunk;
+    const int last_off = sp.back();
+    sp.resize(nslices_pad + 1, last_off);
+    auto dalloc = [&](void **ptr, size_t bytes) {
+        return hipMalloc(ptr, bytes > 0 ? bytes : 16) == hipSuccess;
+    };
+    const size_t nvec = (size_t)pl->nchunks * kChunk;
+    if (!dalloc((void **)&pl->slice_ptr, sizeof(int) * sp.size()) ||
+        !dalloc((void **)&pl->scol, sizeof(int) * sc.size()) ||
+        !dalloc((void **)&pl->sval, sizeof(double) * sv.size()) ||
+        !dalloc((void **)&pl->b, sizeof(double) * nvec) || !dalloc((void **)&pl->x0, sizeof(double) * nvec) ||
+        !dalloc((void **)&pl->x, sizeof(double) * nvec) ||
+        !dalloc((void **)&pl->part, sizeof(double) * 8 * pl->nchunks) ||
+        !dalloc((void **)&pl->scal, sizeof(double) * 2) || !dalloc((void **)&pl->info, sizeof(int) * 4) ||
+        !dalloc((void **)&pl->sync, sizeof(GridSync)))
+        return fail(CFM_ENOMEM);
+    const int nv = p->method == CFM_CG_STANDARD ? 3 : p->method == CFM_CG_PIPELINED ? 6 : 1;
+    for (int i = 0; i < nv; ++i)
+        if (!dalloc((void **)&pl->v[i], sizeof(double) * nvec)) return fail(CFM_ENOMEM);
+    if (p->method == CFM_CG_IR_F32) {
+        for (int i = 0; i < 4; ++i)
+            if (!dalloc((void **)&pl->f[i], sizeof(float) * nvec)) return fail(CFM_ENOMEM);
+        if (!dalloc((void **)&pl->sval32, sizeof(float) * sv.size())) return fail(CFM_ENOMEM);
+        std::vector<float> sv32(sv.size());
+        for (size_t i = 0; i < sv.size(); ++i) sv32[i] = (float)sv[i];
+        if (hipMemcpy(pl->sval32, sv32.data(), sizeof(float) * sv32.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(CFM_EDEVICE);
+    }
+    if (hipHostMalloc((void **)&pl->scal_h, sizeof(double) * 2) != hipSuccess) return fail(CFM_ENOMEM);
+    if (hipMemcpy(pl->slice_ptr, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->scol, sc.data(), sizeof(int) * sc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->sval, sv.data(), sizeof(double) * sv.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(CFM_EDEVICE);
+    if (hipMemset(pl->b, 0, sizeof(double) * nvec) != hipSuccess ||
+        hipMemset(pl->x0, 0, sizeof(double) * nvec) != hipSuccess)
+        return fail(CFM_EDEVICE);
+    if (hipStreamCreate(&pl->stream) != hipSuccess || hipEventCreate(&pl->ev0) != hipSuccess ||
+        hipEventCreate(&pl->ev1) != hipSuccess)
+        return fail(CFM_EDEVICE);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)persistent_kernel(p->method),
+                                                     kThreads, 0) != hipSuccess ||
+        per_cu < 1)
+        return fail(CFM_EDEVICE);
+    const int max_grid = per_cu * pl->dev.cus;
+    int want = p->grid_blocks > 0 ? p->grid_blocks : max_grid;
+    pl->grid = std::max(1, std::min({want, max_grid, pl->nchunks}));
+    int rate_khz = 0;
+    const unsigned long long us = p->spin_limit_us > 0 ? p->spin_limit_us : 10000000ull;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, pl->dev.device) != hipSuccess ||
+        rate_khz <= 0)
+        rate_khz = 100000;
+    pl->spin_ticks = us * (unsigned long long)rate_khz / 1000ull;
+    *out = pl;
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_plan_set_rhs(cfm_cg_plan *pl, const double *b_host) {
+    if (!pl || !b_host) return CFM_EINVAL;
+    CFM_HIP(hipMemcpy(pl->b, b_host, sizeof(double) * pl->n, hipMemcpyHostToDevice));
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_plan_set_x0(cfm_cg_plan *pl, const double *x0_host) {
+    if (!pl) return CFM_EINVAL;
+    if (x0_host)
+        CFM_HIP(hipMemcpy(pl->x0, x0_host, sizeof(double) * pl->n, hipMemcpyHostToDevice));
+    else
+        CFM_HIP(hipMemset(pl->x0, 0, sizeof(double) * pl->n));
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_plan_get_x(cfm_cg_plan *pl, double *x_host) {
+    if (!pl || !x_host) return CFM_EINVAL;
+    CFM_HIP(hipMemcpy(x_host, pl->x, sizeof(double) * pl->n, hipMemcpyDeviceToHost));
+    return CFM_OK;
+}
+
+static CgArgs make_args(cfm_cg_plan *pl) {
+    CgArgs A{};
+    A.n = pl->n;
+    A.nchunks = pl->nchunks;
+    A.A64 = Sell{pl->slice_ptr, pl->scol, pl->sval};
+    A.A32 = Sell{pl->slice_ptr, pl->scol, pl->sval32};
+    A.b = pl->b;
+    A.x0 = pl->x0;
+    A.x = pl->x;
+    for (int i = 0; i < 6; ++i) A.v[i] = pl->v[i];
+    for (int i = 0; i < 4; ++i) A.f[i] = pl->f[i];
+    A.part = pl->part;
+    A.hist = pl->hist;
+    A.hist_cap = pl->hist_cap;
+    A.max_iter = pl->p.max_iter;
+    A.tol = pl->p.tol;
+    A.quirk = pl->p.restart_quirk;
+    A.max_outer = pl->p.max_outer;
+    A.inner_tol = pl->p.inner_tol;
+    A.info = pl->info;
+    A.sync = pl->sync;
+    return A;
+}
+
+// Host-driven schedule of the same phases (reference "Baseline Discrete").
+static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &hist, int *iters_out,
+                               int *status_out) {
+    const dim3 grid(pl->grid), block(kThreads);
+    hipStream_t s = pl->stream;
+    auto fetch = [&](int i0, int i1, double *a, double *b) -> cfm_status {
+        hipLaunchKernelGGL(k_reduce, dim3(1), block, 0, s, A, i0, i1, pl->scal);
+        CFM_HIP(hipGetLastError());
+        CFM_HIP(hipMemcpyAsync(pl->scal_h, pl->scal, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+        CFM_HIP(hipStreamSynchronize(s));
+        *a = pl->scal_h[0];
+        if (b) *b = pl->scal_h[1];
+        return CFM_OK;
+    };
+    const int max_iter = pl->p.max_iter;
+    const double tol = pl->p.tol;
+    int k = 0, status = CFM_OK;
+    bool converged = false;
+    cfm_status st;
+    if (pl->p.method == CFM_CG_STANDARD) {
+        hipLaunchKernelGGL(k_std_init, grid, block, 0, s, A);
+        double gamma, bb;
+        if ((st = fetch(0, 2, &gamma, &bb)) != CFM_OK) return st;
+        const double thr = tol > 0.0 ? tol * tol * bb : 0.0;
+        hist.push_back(gamma);
+        converged = gamma <= thr;
+        while (!converged && k < max_iter) {
+            hipLaunchKernelGGL(k_std_spmv_dot, grid, block, 0, s, A);
+            double delta;
+            if ((st = fetch(1, -1, &delta, nullptr)) != CFM_OK) return st;
+            if (!(delta > 0.0) || !std::isfinite(delta)) {
+                status = CFM_EBREAKDOWN;
+                break;
+            }
+            const double alpha = gamma / delta;
+            hipLaunchKernelGGL(k_std_update_xr, grid, block, 0, s, A, alpha);
+            double gamma_new;
+            if ((st = fetch(0, -1, &gamma_new, nullptr)) != CFM_OK) return st;
+            ++k;
+            hist.push_back(gamma_new);
+            if (gamma_new <= thr) {
+                converged = true;
+                break;
+            }
+            const double beta = gamma_new / gamma;
+            hipLaunchKernelGGL(k_std_update_p, grid, block, 0, s, A, beta);
+            gamma = gamma_new;
+        }
+    } else {
+        hipLaunchKernelGGL(k_pipe_init_r, grid, block, 0, s, A);
+        hipLaunchKernelGGL(k_pipe_init_w, grid, block, 0, s, A);
+        double bb, dummy;
+        if ((st = fetch(4, -1, &bb, nullptr)) != CFM_OK) return st;
+        (void)dummy;
+        const double thr = tol > 0.0 ? tol * tol * bb : 0.0;
+        double alpha = 0.0, beta = 0.0, delta_prev = 0.0;
+        for (;;) {
+            const int buf = k & 1;
+            double delta, gam;
+            if ((st = fetch(0 + buf, 2 + buf, &delta, &gam)) != CFM_OK) return st;
+            hist.push_back(delta);
+            if (delta <= thr) {
+                converged = true;
+                break;
+            }
+            if (k >= max_iter) break;
+            double denom;
+            if (pl->p.restart_quirk ? (k > 1) : (k > 0)) {
+                beta = delta / delta_prev;
+                denom = gam - (beta / alpha) * delta;
+            } else {
+                beta = 0.0;
+                denom = gam;
+            }
+            if (!(denom > 0.0) || !std::isfinite(denom)) {
+                status = CFM_EBREAKDOWN;
+                break;
+            }
+            alpha = delta / denom;
+            hipLaunchKernelGGL(k_pipe_iterate, grid, block, 0, s, A, buf, alpha, beta);
+            delta_prev = delta;
+            ++k;
+        }
+    }
+    CFM_HIP(hipGetLastError());
+    if (status == CFM_OK && !converged && tol > 0.0) status = CFM_ENOTCONV;
+    *iters_out = k;
+    *status_out = status;
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_cap, int *iters,
+                                        cfm_timing *t) {
+    if (!pl || hist_cap < 0) return CFM_EINVAL;
+    const auto w0 = std::chrono::steady_clock::now();
+    // history buffer sized to the request
+    const int cap = hist ? hist_cap : 0;
+    if (cap > pl->hist_cap) {
+        if (pl->hist) (void)hipFree(pl->hist);
+        pl->hist = nullptr;
+        pl->hist_cap = 0;
+        CFM_HIP(hipMalloc((void **)&pl->hist, sizeof(double) * cap));
+        pl->hist_cap = cap;
+    }
+    CgArgs A = make_args(pl);
+    A.hist_cap = cap;
+    if (!hist) A.hist = nullptr;
+    int status = CFM_OK, k = 0, outer = 0;
+    float ms = 0.f;
+    if (pl->p.mode == CFM_CG_PERSISTENT) {
+        hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, pl->stream, pl->sync, pl->spin_ticks);
+        CFM_HIP(hipGetLastError());
+        CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
+        void *args[] = {&A};
+        CFM_HIP(hipLaunchCooperativeKernel((const void *)persistent_kernel(pl->p.method), dim3(pl->grid),
+                                           dim3(kThreads), args, 0, pl->stream));
+        CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
+        CFM_HIP(hipEventSynchronize(pl->ev1));
+        CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+        GridSync hs{};
+        CFM_HIP(hipMemcpy(&hs, pl->sync, sizeof(hs), hipMemcpyDeviceToHost));
+        if (hs.abort) return CFM_ETIMEOUT;
+        int info[4] = {0, 0, 0, 0};
+        CFM_HIP(hipMemcpy(info, pl->info, sizeof(int) * 4, hipMemcpyDeviceToHost));
+        status = info[0];
+        k = info[1];
+        outer = info[2];
+        if (hist && cap > 0) {
+            const int nh = pl->p.method == CFM_CG_IR_F32 ? outer + 1 : k + 1;
+            CFM_HIP(hipMemcpy(hist, pl->hist, sizeof(double) * std::min(cap, nh), hipMemcpyDeviceToHost));
+        }
+    } else {
+        std::vector<double> h;
+        CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
+        cfm_status st = run_discrete(pl, A, h, &k, &status);
+        if (st != CFM_OK) return st;
+        CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
+        CFM_HIP(hipEventSynchronize(pl->ev1));
+        CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+        if (hist)
+            for (int i = 0; i < std::min<int>(cap, (int)h.size()); ++i) hist[i] = h[i];
+    }
+    if (iters) *iters = k;
+    if (t) {
+        t->kernel_ms = ms;
+        t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        t->grid_blocks = pl->grid;
+        t->block_threads = kThreads;
+        t->iterations = k;
+        t->outer_iterations = outer;
+    }
+    return (cfm_status)status;
+}
+
+extern "C" cfm_status cfm_cg_solve(int n, int nnz, const int *rowptr, const int *col, const double *val,
+                                   int index_base, const double *b, double *x_io, const cfm_cg_params *p,
+                                   double *hist, int hist_cap, int *iters, cfm_timing *t) {
+    if (!b || !x_io) return CFM_EINVAL;
+    cfm_cg_plan *pl = nullptr;
+    cfm_status st = cfm_cg_plan_create(n, nnz, rowptr, col, val, index_base, 0, p, &pl);
+    if (st != CFM_OK) return st;
+    st = cfm_cg_plan_set_rhs(pl, b);
+    if (st == CFM_OK) st = cfm_cg_plan_set_x0(pl, x_io);
+    cfm_status solve_st = CFM_OK;
+    if (st == CFM_OK) {
+        solve_st = cfm_cg_plan_solve(pl, hist, hist_cap, iters, t);
+        if (solve_st == CFM_OK || solve_st == CFM_ENOTCONV || solve_st == CFM_EBREAKDOWN)
+            st = cfm_cg_plan_get_x(pl, x_io);
+        else
+            st = solve_st;
+    }
+    cfm_cg_plan_destroy(pl);
+    return st != CFM_OK ? st : solve_st;
+}

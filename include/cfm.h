@@ -1,0 +1,183 @@
+/*
+ * cfm.h -- C ABI of the MI355X-native CPU-free iterative solvers.
+ *
+ * Plain C types only (pointers, sizes, ints, doubles); no C++ or torch types
+ * cross this boundary, nothing here throws, and no entry point calls exit().
+ * One GPU per plan.  Plans are not thread-safe; distinct plans are.
+ * Host pointers are always owned by the caller; device buffers are owned by
+ * the library unless a comment says "borrowed".
+ *
+ * The reference's operator API is its version table: each version exports
+ *     int init(int argc, char **argv)                      (stencils)
+ *     int init(int *rowptr_d, int *col_d, real *val_d, int n, int nnz,
+ *              bool zero_indexed, int iter_max, real *x_final, ...)  (CG)
+ * selected by `-v` (Stencil/jacobi2D/src/main.cu:25-68,
+ * Stencil/jacobi3D/src/main.cu:31-72, CG/src/main.cu:24-35,
+ * CG/include/common.h:122-133).  Each of those functions parses flags,
+ * allocates, launches, times and reports; this ABI splits that work into
+ * plan / load / run / download steps so callers can time the kernel alone.
+ * The CLI binaries jacobi2D, jacobi3D and cg rebuild the version tables on top.
+ */
+#ifndef CFM_H
+#define CFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CFM_OK = 0,
+    CFM_EINVAL = 1,     /* bad argument or unsupported shape */
+    CFM_ENOMEM = 2,     /* device or host allocation failed */
+    CFM_EDEVICE = 3,    /* no usable GPU, launch or runtime failure */
+    CFM_ETIMEOUT = 4,   /* a device-wide barrier exceeded its spin limit */
+    CFM_ENOTCONV = 5,   /* CG reached max_iter above the tolerance */
+    CFM_EBREAKDOWN = 6, /* CG denominator <= 0 or not finite (matrix not SPD) */
+    CFM_ENOTSUP = 7
+} cfm_status;
+
+/* Element type of the Jacobi sweep.  CFM_F32 sweeps in FP32 and always
+ * accumulates the update norm in FP64. */
+enum { CFM_F64 = 0, CFM_F32 = 1 };
+
+/* Handling of the slowest axis (y in 2D, z in 3D).
+ * CFM_BC_PERIODIC: after each sweep the last interior row/plane is copied
+ *   to index 0 and the first interior row/plane to index n-1, as the
+ *   reference's single-GPU loop does (Stencil/jacobi2D/src/common.cu:234-242,
+ *   Stencil/jacobi3D/src/common.cu:198-207).
+ * CFM_BC_DIRICHLET: every face keeps its initial value. */
+enum { CFM_BC_PERIODIC = 0, CFM_BC_DIRICHLET = 1 };
+
+/* Execution scheme of the Jacobi sweep loop. */
+enum {
+    CFM_JACOBI_LAUNCH_PER_ITER = 0, /* host loop, one launch per sweep ("Baseline Copy") */
+    CFM_JACOBI_PERSISTENT = 1,      /* one cooperative launch, device-wide barrier per sweep ("Design 2") */
+    CFM_JACOBI_NO_COMPUTE = 2       /* persistent kernel doing only the barriers ("No computation") */
+};
+
+/* Built-in initial fields (computed on the host with the C library's sin). */
+enum {
+    CFM_INIT_SIN = 0,      /* 2D: a[y][0]=a[y][nx-1]=sin(2 pi y/(ny-1)); 3D: both x faces
+                              sin(2 pi y/(ny-1)) * sin(2 pi z/(nz-1)); everything else 0 */
+    CFM_INIT_REF_QUIRK = 1 /* 3D: x=0 face = (z - y) - (nx-1), the value the reference's
+                              P2P build leaves (Stencil/jacobi3D/src/common.cu:15-29) */
+};
+
+typedef struct {
+    int nx, ny, nz;   /* nz == 1 selects the 2D 5-point stencil; dims >= 3 */
+    int niter;        /* sweeps per run, >= 0 */
+    int norm_every;   /* > 0: L2 norm of the update on iterations with iter % norm_every == 0 */
+    int prec;         /* CFM_F64 or CFM_F32 */
+    int bc;           /* CFM_BC_* */
+    int variant;      /* CFM_JACOBI_* */
+    int tile_rows;    /* rows per wavefront tile (3D: y rows, 2D: rows per prefetch group); 0 = auto */
+    int grid_blocks;  /* persistent grid size; 0 = auto (all co-resident workgroups) */
+    int chunks;       /* chunks along the streamed axis; 0 = auto */
+    int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
+} cfm_jacobi_params;
+
+typedef struct {
+    double kernel_ms;   /* HIP-event time of the launches on the plan's stream */
+    double wall_ms;     /* host wall time of the run call */
+    int grid_blocks;    /* workgroups launched */
+    int block_threads;  /* threads per workgroup */
+    int iterations;     /* sweeps (Jacobi) or CG iterations (summed inner for IR) */
+    int outer_iterations;
+} cfm_timing;
+
+typedef struct cfm_jacobi_plan cfm_jacobi_plan;
+typedef struct cfm_cg_plan cfm_cg_plan;
+
+/* ------------------------------------------------------------ library --- */
+const char *cfm_version(void);
+const char *cfm_status_str(int status);
+cfm_status cfm_device_count(int *count);
+/* name, compute units, HBM bytes of device `dev` */
+cfm_status cfm_device_info(int dev, char *name, int name_cap, int *compute_units,
+                           size_t *mem_bytes);
+cfm_status cfm_device_synchronize(void);
+
+/* ------------------------------------------------------------- Jacobi --- */
+/* Replaces the allocate/initialise/launch/swap part of
+ * SSMultiThreadedTwoBlockComm::init (Stencil/jacobi2D/src/single-stream/
+ * multi-threaded-two-block-comm.cu:126-308, Stencil/jacobi3D/src/single-stream/
+ * multi-threaded-two-block-comm.cu:126-336) and of single_gpu
+ * (Stencil/jacobi2D/src/common.cu:154-273, Stencil/jacobi3D/src/common.cu:110-239). */
+cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jacobi_plan **out);
+/* Copy an nz*ny*nx field of the plan's element type into both working arrays. */
+cfm_status cfm_jacobi_plan_load(cfm_jacobi_plan *plan, const void *init_host);
+/* Advance the field by niter sweeps.  norms (may be NULL) receives up to
+ * norm_cap update norms; norm_count the number produced. */
+cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *plan, double *norms, int norm_cap,
+                               int *norm_count, cfm_timing *t);
+cfm_status cfm_jacobi_plan_download(cfm_jacobi_plan *plan, void *out_host);
+/* Borrowed device pointer to the current field (valid until the next run). */
+cfm_status cfm_jacobi_plan_field(cfm_jacobi_plan *plan, void **device_ptr);
+void cfm_jacobi_plan_destroy(cfm_jacobi_plan *plan);
+/* One shot: create, load, run, download, destroy. */
+cfm_status cfm_jacobi_run(const cfm_jacobi_params *p, const void *init_host, void *out_host,
+                          double *norms, int norm_cap, int *norm_count, cfm_timing *t);
+/* Host generator of the built-in initial fields (prec selects double/float output).
+ * Replaces initialize_boundaries (Stencil/jacobi2D/src/common.cu:15-26,
+ * Stencil/jacobi3D/src/common.cu:15-29). */
+cfm_status cfm_jacobi_init_field(int nx, int ny, int nz, int kind, int prec, void *out_host);
+
+/* ------------------------------------------------------------------ CG --- */
+enum { CFM_CG_STANDARD = 0, CFM_CG_PIPELINED = 1, CFM_CG_IR_F32 = 2 };
+enum {
+    CFM_CG_PERSISTENT = 0, /* whole solve in one cooperative launch ("Single Stream") */
+    CFM_CG_DISCRETE = 1    /* one launch per operation + host scalar reads ("Baseline Discrete") */
+};
+
+typedef struct {
+    int method;          /* CFM_CG_* */
+    int mode;            /* CFM_CG_PERSISTENT or CFM_CG_DISCRETE */
+    double tol;          /* > 0: stop at <r,r> <= tol^2 <b,b>; <= 0: exactly max_iter iterations */
+    int max_iter;        /* CG iterations (IR: inner iterations per refinement) */
+    int restart_quirk;   /* pipelined: 1 = reference's `k > 1` restart (default), 0 = `k > 0` */
+    int max_outer;       /* IR: refinement steps (0 = 50) */
+    double inner_tol;    /* IR: inner relative tolerance (0 = 1e-4) */
+    int grid_blocks;     /* 0 = auto */
+    int spin_limit_us;   /* device barrier watchdog; 0 = default (10 s) */
+} cfm_cg_params;
+
+/* Replaces SingleStreamStandard::init / SingleStreamPipelined::init and
+ * SingleGPUDiscreteStandard::run_single_gpu (CG/include/common.h:117-133,
+ * CG/src/single-stream/standard.cu:302-434, CG/src/single-stream/pipelined.cu:321-465,
+ * CG/src/common.cu:399-538).  The CSR (int32 indices, float64 values,
+ * index_base 0 or 1 as the reference's zero_indexed flag) is read from host
+ * memory (csr_on_device = 0) or from borrowed device memory (1) and stored
+ * on the device in the library's own sliced layout. */
+cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, const int *col,
+                              const double *val, int index_base, int csr_on_device,
+                              const cfm_cg_params *p, cfm_cg_plan **out);
+cfm_status cfm_cg_plan_set_rhs(cfm_cg_plan *plan, const double *b_host);
+/* x0 (NULL = zero vector); each solve starts from the stored x0. */
+cfm_status cfm_cg_plan_set_x0(cfm_cg_plan *plan, const double *x0_host);
+/* Solve from device-resident b and x0.  hist (may be NULL, cap hist_cap)
+ * receives <r_k,r_k> for k = 0..iters (IR: the outer residuals). */
+cfm_status cfm_cg_plan_solve(cfm_cg_plan *plan, double *hist, int hist_cap, int *iters,
+                             cfm_timing *t);
+cfm_status cfm_cg_plan_get_x(cfm_cg_plan *plan, double *x_host);
+void cfm_cg_plan_destroy(cfm_cg_plan *plan);
+/* One shot on host arrays. x_io: x0 in, solution out. */
+cfm_status cfm_cg_solve(int n, int nnz, const int *rowptr, const int *col, const double *val,
+                        int index_base, const double *b, double *x_io, const cfm_cg_params *p,
+                        double *hist, int hist_cap, int *iters, cfm_timing *t);
+
+/* ---------------------------------------------------------- generators --- */
+/* 3D 7-point Poisson on an nx*ny*nz grid, Dirichlet, rows ix + nx*(iy + ny*iz),
+ * columns ascending, diagonal 6, off-diagonals -1 (the CG workload; the
+ * reference generates a random tridiagonal instead, CG/src/common.cu:46-76).
+ * Returns nnz (>= 0) or -1; pass NULL arrays to query the size. */
+int64_t cfm_gen_poisson3d(int nx, int ny, int nz, int *rowptr, int *col, double *val);
+/* SplitMix64 stream (seed, stream_id) -> count doubles in [0,1). */
+cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFM_H */

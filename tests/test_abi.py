@@ -1,0 +1,84 @@
+"""The C ABI (include/cfm.h) without a GPU: the library loads, exports every
+declared symbol, validates arguments, and its host-side generators agree with
+the oracle bit for bit."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+
+def declared_symbols():
+    src = open(os.path.join(REPO, "include", "cfm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cfm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_declared_symbol_is_exported(cfm):
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(cfm.lib, name), name
+    assert set(names) == set(cfm.EXPORTS)
+
+
+def test_version_and_status_strings(cfm):
+    assert "gfx950" in cfm.version()
+    assert cfm.status_str(cfm.OK) == "ok"
+    assert "converged" in cfm.status_str(cfm.ENOTCONV)
+    assert cfm.status_str(99) == "unknown status"
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4), (7, 3, 5), (128, 2, 1)])
+def test_poisson_generator_matches_oracle(cfm, orc, dims):
+    rp, col, val = cfm.poisson3d(*dims)
+    A = orc.poisson3d(*dims)
+    assert np.array_equal(rp, A.rowptr) and np.array_equal(col, A.col) and np.array_equal(val, A.val)
+
+
+def test_uniform_generator_matches_oracle(cfm, orc):
+    for seed, sid in ((0, 0), (12345, 7), (2**63 + 5, 255)):
+        assert np.array_equal(cfm.uniform(seed, sid, 4096), orc.uniform(seed, sid, 4096))
+
+
+def test_init_fields_match_oracle(cfm, orc):
+    assert np.array_equal(cfm.init_field(19, 13), orc.init2d(19, 13))
+    assert np.array_equal(cfm.init_field(9, 8, 7), orc.init3d(9, 8, 7))
+    assert np.array_equal(cfm.init_field(9, 8, 7, cfm.INIT_REF_QUIRK), orc.init3d(9, 8, 7, orc.INIT_REF_QUIRK))
+    f32 = cfm.init_field(9, 8, 7, prec=cfm.F32)
+    assert f32.dtype == np.float32 and np.array_equal(f32, orc.init3d(9, 8, 7).astype(np.float32))
+
+
+def test_argument_validation(cfm):
+    bad = [dict(nx=2, ny=5), dict(nx=5, ny=5, nz=2), dict(nx=5, ny=5, niter=-1), dict(nx=5, ny=5, prec=7),
+           dict(nx=5, ny=5, bc=3), dict(nx=5, ny=5, variant=9)]
+    for kw in bad:
+        p = cfm.jacobi_params(**kw)
+        h = ctypes.c_void_p()
+        assert cfm.lib.cfm_jacobi_plan_create(ctypes.byref(p), ctypes.byref(h)) == cfm.EINVAL, kw
+        assert not h.value
+    assert cfm.lib.cfm_jacobi_plan_create(None, None) == cfm.EINVAL
+    assert cfm.lib.cfm_jacobi_init_field(2, 5, 1, 0, 0, None) == cfm.EINVAL
+    rp = np.array([0, 1], dtype=np.int32)
+    col = np.array([0], dtype=np.int32)
+    val = np.array([1.0])
+    p = cfm.cg_params(method=9)
+    h = ctypes.c_void_p()
+    assert cfm.lib.cfm_cg_plan_create(1, 1, rp.ctypes.data, col.ctypes.data, val.ctypes.data, 0, 0,
+                                      ctypes.byref(p), ctypes.byref(h)) == cfm.EINVAL
+    p = cfm.cg_params(method=cfm.CG_IR_F32, mode=cfm.CG_DISCRETE)
+    assert cfm.lib.cfm_cg_plan_create(1, 1, rp.ctypes.data, col.ctypes.data, val.ctypes.data, 0, 0,
+                                      ctypes.byref(p), ctypes.byref(h)) == cfm.ENOTSUP
+
+
+def test_no_device_is_an_error_not_a_fallback(cfm):
+    if cfm.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    p = cfm.jacobi_params(8, 8, 8, niter=1)
+    h = ctypes.c_void_p()
+    assert cfm.lib.cfm_jacobi_plan_create(ctypes.byref(p), ctypes.byref(h)) == cfm.EDEVICE
+    with pytest.raises(cfm.CfmError):
+        cfm.jacobi(np.zeros((8, 8, 8)), 1)

@@ -1,0 +1,133 @@
+"""Jacobi parity on the GPU: the HIP kernels, called through the C ABI, must
+equal the CPU oracle bit for bit (FP64 and FP32 fields; same left-to-right
+sums as Stencil/jacobi2D/src/common.cu:37 and Stencil/jacobi3D/src/common.cu:43),
+for both boundary modes, both execution schemes and every tile shape.  The
+update norms are sums in a different order: relative tolerance 1e-10.
+At the full benchmark sizes (2D 16384^2, 3D 512^3) the checks are the
+size-independent properties: mirror symmetry in x (bitwise), the discrete
+maximum principle, a constant field staying constant, and norms decreasing."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NORM_RTOL = 1e-10
+
+
+def _rand(shape, dtype, seed):
+    return np.random.default_rng(seed).standard_normal(shape).astype(dtype)
+
+
+def _check(gpu, orc, init, niter, bc, norm_every=0, variant=None, **tuning):
+    variant = gpu.PERSISTENT if variant is None else variant
+    out, norms, t = gpu.jacobi(init, niter, bc=bc, norm_every=norm_every, variant=variant, **tuning)
+    ref, rnorms = orc.jacobi(init, niter, bc, norm_every)
+    assert out.dtype == init.dtype
+    bad = np.argwhere(out != ref)
+    assert bad.size == 0, f"{len(bad)} mismatches, first at {bad[:3].tolist()}"
+    assert len(norms) == len(rnorms)
+    if len(norms):
+        assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
+    return t
+
+
+SHAPES_2D = [(11, 18), (35, 67), (9, 130), (5, 300)]
+SHAPES_3D = [(7, 9, 10), (19, 35, 67), (6, 9, 130), (34, 34, 34)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", SHAPES_2D + SHAPES_3D)
+@pytest.mark.parametrize("bc", [0, 1])
+@pytest.mark.parametrize("niter", [0, 1, 2, 7])
+def test_bitexact_persistent(gpu, orc, dtype, shape, bc, niter):
+    init = _rand(shape, dtype, hash((shape, niter)) & 0xFFFF)
+    _check(gpu, orc, init, niter, bc, norm_every=3)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(35, 67), (19, 35, 67)])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_bitexact_launch_per_iter(gpu, orc, dtype, shape, bc):
+    init = _rand(shape, dtype, 5)
+    _check(gpu, orc, init, 7, bc, norm_every=2, variant=gpu.LAUNCH_PER_ITER)
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4, 8])
+@pytest.mark.parametrize("grid", [0, 1, 3])
+def test_tile_shapes_3d(gpu, orc, rows, grid):
+    init = _rand((21, 37, 150), np.float64, rows * 10 + grid)
+    _check(gpu, orc, init, 5, 0, norm_every=1, tile_rows=rows, grid_blocks=grid)
+
+
+@pytest.mark.parametrize("rows", [4, 8, 16])
+@pytest.mark.parametrize("grid", [0, 1, 3])
+@pytest.mark.parametrize("chunks", [0, 1, 7])
+def test_tile_shapes_2d(gpu, orc, rows, grid, chunks):
+    init = _rand((203, 300), np.float64, rows + grid + chunks)
+    _check(gpu, orc, init, 4, 0, norm_every=1, tile_rows=rows, grid_blocks=grid, chunks=chunks)
+
+
+def test_reference_quirk_init_3d(gpu, orc):
+    init = orc.init3d(40, 33, 20, orc.INIT_REF_QUIRK)
+    _check(gpu, orc, init, 9, 0)
+
+
+def test_config_init_periodic_and_dirichlet(gpu, orc):
+    for bc in (0, 1):
+        _check(gpu, orc, orc.init2d(130, 70), 25, bc, norm_every=5)
+        _check(gpu, orc, orc.init3d(66, 40, 30), 12, bc, norm_every=4)
+
+
+def test_plan_runs_compose(gpu, orc):
+    init = _rand((20, 30, 70), np.float64, 9)
+    p = gpu.jacobi_params(70, 30, 20, niter=3, bc=0)
+    with gpu.JacobiPlan(p) as plan:
+        plan.load(init)
+        plan.run()
+        plan.run()
+        out = plan.download()
+    ref, _ = orc.jacobi(init, 6, 0)
+    assert np.array_equal(out, ref)
+
+
+def test_no_compute_variant_leaves_field(gpu):
+    init = _rand((10, 20, 30), np.float64, 1)
+    out, norms, t = gpu.jacobi(init, 50, variant=gpu.NO_COMPUTE)
+    assert np.array_equal(out, init)
+
+
+def test_barrier_watchdog_reports_timeout(gpu):
+    # a 1-microsecond spin limit makes the device barrier give up: every
+    # workgroup leaves at its next barrier and the run reports ETIMEOUT
+    init = np.zeros((64, 256, 256))
+    with pytest.raises(gpu.CfmError) as e:
+        gpu.jacobi(init, 200, spin_limit_us=1)
+    assert e.value.status == gpu.ETIMEOUT
+    # the library is still usable afterwards
+    out, _, _ = gpu.jacobi(np.ones((8, 9, 10)), 3)
+    assert np.all(out == 1.0)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_full_size_3d_properties(gpu, dtype):
+    n = 512
+    prec = gpu.F64 if dtype == np.float64 else gpu.F32
+    init = gpu.init_field(n, n, n, prec=prec)
+    out, norms, t = gpu.jacobi(init, 1000, norm_every=100)
+    assert np.array_equal(out, out[:, :, ::-1])
+    inner = out[1:-1, 1:-1, 1:-1]
+    assert np.isfinite(inner).all()
+    assert inner.min() >= init.min() and inner.max() <= init.max()
+    assert np.all(np.diff(norms) < 0)
+    ones = np.ones((n, n, n), dtype=dtype)
+    out1, _, _ = gpu.jacobi(ones, 10)
+    assert np.all(out1 == 1)
+
+
+def test_full_size_2d_properties(gpu):
+    n = 16384
+    init = gpu.init_field(n, n)
+    out, norms, t = gpu.jacobi(init, 1000, norm_every=100)
+    assert np.array_equal(out, out[:, ::-1])
+    assert out[1:-1, 1:-1].min() >= init.min() and out[1:-1, 1:-1].max() <= init.max()
+    assert np.all(np.diff(norms) < 0)

@@ -30,6 +30,7 @@ CG_PERSISTENT, CG_DISCRETE = 0, 1
 # Every symbol include/cfm.h declares.
 EXPORTS = (
     "cfm_version", "cfm_status_str", "cfm_device_count", "cfm_device_info", "cfm_device_synchronize",
+    "cfm_bandwidth_probe",
     "cfm_jacobi_plan_create", "cfm_jacobi_plan_load", "cfm_jacobi_plan_run", "cfm_jacobi_plan_download",
     "cfm_jacobi_plan_field", "cfm_jacobi_plan_destroy", "cfm_jacobi_run", "cfm_jacobi_init_field",
     "cfm_cg_plan_create", "cfm_cg_plan_set_rhs", "cfm_cg_plan_set_x0", "cfm_cg_plan_solve",
@@ -46,7 +47,7 @@ class CfmError(RuntimeError):
 class JacobiParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "nx", "ny", "nz", "niter", "norm_every", "prec", "bc", "variant", "tile_rows", "grid_blocks",
-        "chunks", "spin_limit_us")]
+        "chunks", "spin_limit_us", "block_threads")]
 
 
 class Timing(ctypes.Structure):
@@ -62,7 +63,7 @@ class CgParams(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("mode", ctypes.c_int), ("tol", ctypes.c_double),
                 ("max_iter", ctypes.c_int), ("restart_quirk", ctypes.c_int), ("max_outer", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("grid_blocks", ctypes.c_int),
-                ("spin_limit_us", ctypes.c_int)]
+                ("spin_limit_us", ctypes.c_int), ("block_threads", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -80,6 +81,7 @@ def _load():
     L.cfm_device_count.argtypes = [_ip]
     L.cfm_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _ip,
                                   ctypes.POINTER(ctypes.c_size_t)]
+    L.cfm_bandwidth_probe.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp, _dp]
     L.cfm_jacobi_plan_create.argtypes = [ctypes.POINTER(JacobiParams), ctypes.POINTER(_vp)]
     L.cfm_jacobi_plan_load.argtypes = [_vp, _vp]
     L.cfm_jacobi_plan_run.argtypes = [_vp, _dp, ctypes.c_int, _ip, ctypes.POINTER(Timing)]
@@ -136,6 +138,13 @@ def device_info(dev=0):
     return {"name": name.value.decode(), "compute_units": cus.value, "mem_bytes": mem.value}
 
 
+def bandwidth_probe(nbytes=1 << 30, reps=5):
+    """Best copy bandwidth (GB/s, read + write) of a plain streaming copy."""
+    gbs, ms = ctypes.c_double(0), ctypes.c_double(0)
+    _check(lib.cfm_bandwidth_probe(nbytes, reps, ctypes.byref(gbs), ctypes.byref(ms)), "cfm_bandwidth_probe")
+    return gbs.value, ms.value
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
@@ -150,9 +159,9 @@ def init_field(nx, ny, nz=1, kind=INIT_SIN, prec=F64):
 
 
 def jacobi_params(nx, ny, nz=1, niter=1000, norm_every=0, prec=F64, bc=BC_DIRICHLET,
-                  variant=PERSISTENT, tile_rows=0, grid_blocks=0, chunks=0, spin_limit_us=0):
+                  variant=PERSISTENT, tile_rows=0, grid_blocks=0, chunks=0, spin_limit_us=0, block_threads=0):
     return JacobiParams(nx, ny, nz, niter, norm_every, prec, bc, variant, tile_rows, grid_blocks, chunks,
-                        spin_limit_us)
+                        spin_limit_us, block_threads)
 
 
 class JacobiPlan:
@@ -247,9 +256,9 @@ def uniform(seed, stream_id, count):
 
 
 def cg_params(method=CG_STANDARD, mode=CG_PERSISTENT, tol=1e-8, max_iter=10000, restart_quirk=1,
-              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0):
+              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0, block_threads=0):
     return CgParams(method, mode, tol, max_iter, restart_quirk, max_outer, inner_tol, grid_blocks,
-                    spin_limit_us)
+                    spin_limit_us, block_threads)
 
 
 class CgPlan:

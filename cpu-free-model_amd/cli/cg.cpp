@@ -128,6 +128,8 @@ int main(int argc, char *argv[]) {
     const unsigned long long seed = get_argval<unsigned long long>(b, e, "-seed", 0ull);
     const std::string rhs = get_argval<std::string>(b, e, "-rhs", "uniform");
     const int quirk = get_argval<int>(b, e, "-quirk", 1);
+    const int grid = get_argval<int>(b, e, "-grid", 0);
+    const int block = get_argval<int>(b, e, "-block", 0);
 
     std::vector<int> versions;
     {
@@ -163,6 +165,8 @@ int main(int argc, char *argv[]) {
         p.tol = tol;
         p.max_iter = iter_max;
         p.restart_quirk = quirk;
+        p.grid_blocks = grid;
+        p.block_threads = block;
         x.assign(A.n, 0.0);
         hist.assign(iter_max + 2, 0.0);
         return cfm_cg_solve(A.n, nnz, A.rowptr.data(), A.col.data(), A.val.data(), 0, bvec.data(), x.data(), &p,
@@ -211,8 +215,10 @@ int main(int argc, char *argv[]) {
                 const int nh = methods[v] == CFM_CG_IR_F32 ? t.outer_iterations : iters;
                 double bb = 0.0;
                 for (double q : bvec) bb += q * q;
-                std::printf("iterations: %d (outer %d), relative residual: %.3e, kernel: %.4f ms, status: %s\n",
-                            iters, t.outer_iterations, std::sqrt(hist[nh] / bb), t.kernel_ms, cfm_status_str(st));
+                std::printf("iterations: %d (outer %d), relative residual: %.3e, kernel: %.4f ms, grid: %d x %d, "
+                            "status: %s\n",
+                            iters, t.outer_iterations, std::sqrt(hist[nh] / bb), t.kernel_ms, t.grid_blocks,
+                            t.block_threads, cfm_status_str(st));
             } else {
                 return 1;
             }

@@ -54,6 +54,7 @@ inline int run_jacobi(int argc, char **argv, int dims) {
     p.tile_rows = get_argval<int>(b, e, "-tile", 0);
     p.grid_blocks = get_argval<int>(b, e, "-grid", 0);
     p.chunks = get_argval<int>(b, e, "-chunks", 0);
+    p.block_threads = get_argval<int>(b, e, "-block", 0);
     const bool json = get_arg(b, e, "-json");
     p.prec = prec == "f32" ? CFM_F32 : CFM_F64;
     p.bc = bc == "dirichlet" ? CFM_BC_DIRICHLET : CFM_BC_PERIODIC;
@@ -137,9 +138,10 @@ inline int run_jacobi(int argc, char **argv, int dims) {
     }
     if (json) {
         const double lups = (double)(p.nx - 2) * (p.ny - 2) * (dims == 2 ? 1 : p.nz - 2) * p.niter;
-        std::printf("{\"version\": \"%s\", \"kernel_ms\": %.4f, \"glups\": %.3f, \"gbs\": %.1f, \"grid\": %d}\n",
+        std::printf("{\"version\": \"%s\", \"kernel_ms\": %.4f, \"glups\": %.3f, \"gbs\": %.1f, \"grid\": %d, "
+                    "\"block\": %d}\n",
                     names[selection], t.kernel_ms, lups / (t.kernel_ms * 1e6),
-                    lups * 2 * esz / (t.kernel_ms * 1e6), t.grid_blocks);
+                    lups * 2 * esz / (t.kernel_ms * 1e6), t.grid_blocks, t.block_threads);
     }
     return 0;
 }

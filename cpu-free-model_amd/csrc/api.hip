@@ -49,6 +49,54 @@ extern "C" cfm_status cfm_device_synchronize(void) {
     return CFM_OK;
 }
 
+// ---------------------------------------------------- bandwidth probe ---
+// Plain streaming copy (16 B per lane per step, grid-stride).  It calibrates
+// the hardware counters (FETCH_SIZE / WRITE_SIZE per known byte count) and
+// gives the practical HBM ceiling next to which the solver kernels are read.
+namespace {
+typedef double v2f64_t __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) copy_probe_kernel(const v2f64_t *__restrict__ a,
+                                                         v2f64_t *__restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+}  // namespace
+
+extern "C" cfm_status cfm_bandwidth_probe(size_t bytes, int reps, double *copy_gbs, double *best_ms) {
+    if (bytes < 16 || reps < 1 || !copy_gbs) return CFM_EINVAL;
+    const size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    cfm_status st = CFM_OK;
+    float best = 1e30f;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+        st = CFM_ENOMEM;
+    } else if (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+               hipEventCreate(&e1) != hipSuccess) {
+        st = CFM_EDEVICE;
+    } else {
+        for (int r = 0; r < reps + 1 && st == CFM_OK; ++r) {
+            hipEventRecord(e0, nullptr);
+            hipLaunchKernelGGL(copy_probe_kernel, dim3(1024), dim3(256), 0, nullptr,
+                               static_cast<const v2f64_t *>(a), static_cast<v2f64_t *>(b), n);
+            hipEventRecord(e1, nullptr);
+            float ms = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                st = CFM_EDEVICE;
+            else if (r > 0 && ms < best)
+                best = ms;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (st != CFM_OK) return st;
+    *copy_gbs = 2.0 * (double)(n * 16) / (best * 1e-3) / 1e9;
+    if (best_ms) *best_ms = best;
+    return CFM_OK;
+}
+
 // ------------------------------------------------------- initial fields ---
 template <typename T>
 static void init_field(int nx, int ny, int nz, int kind, T *a) {

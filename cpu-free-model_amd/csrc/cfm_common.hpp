@@ -25,49 +25,79 @@ constexpr int kWave = 64;  // CDNA wavefront width
     } while (0)
 
 // ---------------------------------------------------------------- barrier --
-// Device-wide barrier state for one cooperative launch.  `count` grows by
-// one per workgroup per barrier (never reset inside a launch); barrier number
-// e completes when count reaches e * gridDim.x.  `abort` is raised when a
-// workgroup spins longer than the limit; every workgroup then leaves its loop
-// at its next barrier, so the grid always drains.
+// Device-wide barrier for one cooperative launch, built from plain stores and
+// loads (no atomic read-modify-write: device-scope atomics on one address
+// serialise across the 8 XCDs and cost far more than the barrier itself).
+//   arrive: workgroup b stores its barrier count e into arrive[b] (release);
+//   gather: the 256 threads of workgroup 0 wait until every arrive[] >= e;
+//   release: workgroup 0 stores e into 8 replicated `go` words (one per XCD,
+//            each on its own 128-byte line), every workgroup polls its copy.
+// Every wait is bounded by spin_limit_ticks of the 100 MHz s_memrealtime
+// clock; a workgroup that runs out raises `abort`, and every workgroup leaves
+// its loop at its next barrier, so the grid always drains.
+constexpr int kMaxGrid = 8192;
+constexpr int kGoCopies = 8;
+
 struct GridSync {
-    unsigned long long count;
+    unsigned int go[kGoCopies * 32];  // copy i at go[32 * i]
     unsigned int abort;
-    unsigned int pad0;
-    unsigned long long spin_limit_ticks;  // s_memrealtime ticks (100 MHz)
-    unsigned long long pad1[5];
+    unsigned int pad0[31];
+    unsigned long long spin_limit_ticks;
+    unsigned long long pad1[15];
+    unsigned int arrive[kMaxGrid];
 };
 
-__global__ void reset_sync_kernel(GridSync *s, unsigned long long limit_ticks);
+__global__ void reset_sync_kernel(GridSync *s, unsigned long long limit_ticks, int grid);
+
+__device__ __forceinline__ bool spin_expired(unsigned long long t0, unsigned long long limit) {
+    return __builtin_amdgcn_s_memrealtime() - t0 > limit;
+}
 
 // Returns true when the barrier completed, false when the launch is aborting.
-__device__ __forceinline__ bool grid_barrier(GridSync *s, unsigned long long &epoch) {
+__device__ __forceinline__ bool grid_barrier(GridSync *s, unsigned int &epoch) {
     __shared__ int s_abort;
     __syncthreads();
     epoch += 1;
+    const unsigned int e = epoch;
+    const unsigned long long limit = s->spin_limit_ticks;
     if (threadIdx.x == 0) {
+        s_abort = 0;
+        __hip_atomic_store(&s->arrive[blockIdx.x], e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (blockIdx.x == 0) {
+        // gather: every thread of workgroup 0 watches a strided subset
+        __syncthreads();
         int ab = 0;
-        // Release this workgroup's stores to agent scope (L2 write-back
-        // across XCDs), then arrive.
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(&s->count, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long target = epoch * (unsigned long long)gridDim.x;
         const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-        const unsigned long long limit = s->spin_limit_ticks;
-        while (__hip_atomic_load(&s->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                ab = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            if (__builtin_amdgcn_s_memrealtime() - t0 > limit) {
-                __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ab = 1;
-                break;
+        for (int b = threadIdx.x; b < (int)gridDim.x && !ab; b += blockDim.x) {
+            while (__hip_atomic_load(&s->arrive[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e) {
+                if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    spin_expired(t0, limit)) {
+                    __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_abort = ab;
+        if (ab) s_abort = 1;
+        __syncthreads();
+        if (threadIdx.x < kGoCopies && !s_abort)
+            __hip_atomic_store(&s->go[32 * threadIdx.x], e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        unsigned int *go = &s->go[32 * (blockIdx.x % kGoCopies)];
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        while (__hip_atomic_load(go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e) {
+            if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                spin_expired(t0, limit)) {
+                __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
     return s_abort == 0;
@@ -91,6 +121,20 @@ __device__ __forceinline__ double block256_sum(double v, double *lds4) {
     if ((threadIdx.x & 63) == 0) lds4[w] = v;
     __syncthreads();
     return (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+}
+
+// Sum over the WAVES wavefronts of a workgroup, wave order 0, 1, 2, ...
+// (used for update norms, which carry a tolerance, not a fixed tree).
+template <int WAVES>
+__device__ __forceinline__ double block_sum_waves(double v, double *lds) {
+    v = wave_butterfly(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) s += lds[w];
+    return s;
 }
 
 // ------------------------------------------------------------- launch ----

@@ -97,25 +97,49 @@ __device__ __forceinline__ XT sell_row(const Sell &S, const XT *__restrict__ x, 
     return acc;
 }
 
-// Reduce nchunks partials (fixed order) -> same value in every thread.
-__device__ __forceinline__ double reduce_partials(const double *part, int nchunks, double *lds4) {
+// A workgroup is SUB sub-blocks of 256 threads; every sub-block forms its
+// sums with the same fixed tree: xor butterfly per wavefront, then
+// (w0 + w1) + (w2 + w3) over its four wavefronts.  lds holds 4*SUB doubles.
+template <int SUB>
+__device__ __forceinline__ double sub_sum(double v, double *lds) {
+    v = wave_butterfly(v);
+    __syncthreads();  // lds may still be read from a previous call
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const int w0 = (threadIdx.x >> 8) * 4;
+    return (lds[w0] + lds[w0 + 1]) + (lds[w0 + 2] + lds[w0 + 3]);
+}
+
+// Reduce nchunks partials (fixed order) -> same value in every thread: each
+// sub-block's thread t adds partials t, t+256, ... and the sub-block tree
+// combines them (every sub-block computes the identical value).
+template <int SUB>
+__device__ __forceinline__ double reduce_partials(const double *part, int nchunks, double *lds) {
     double a = 0.0;
-    for (int j = threadIdx.x; j < nchunks; j += kThreads) a = a + part[j];
-    return block256_sum(a, lds4);
+    for (int j = threadIdx.x & (kThreads - 1); j < nchunks; j += kThreads) a = a + part[j];
+    return sub_sum<SUB>(a, lds);
 }
 
 __device__ __forceinline__ bool is_finite(double v) { return v - v == 0.0; }
 
-// Iterate the rows this thread owns: chunk c, rows c*1024 + i*256 + t.
+// Iterate the rows this thread owns: sub-block `sub_` of workgroup b takes
+// chunk c = b*SUB + sub_ (+ multiples of gridDim.x*SUB); inside a chunk its
+// thread t owns rows c*1024 + i*256 + t, i = 0..3 (slice c*16 + i*4 + t/64).
+// The loop trip count is uniform over the workgroup (inactive sub-blocks
+// still take part in the __syncthreads of the chunk sums).
 #define CFM_FOR_MY_ROWS(...)                                                               \
-    for (int c_ = blockIdx.x; c_ < A.nchunks; c_ += gridDim.x) {                            \
+    const int sub_ = threadIdx.x >> 8;                                                     \
+    const int t_ = threadIdx.x & (kThreads - 1);                                           \
+    for (int base_ = blockIdx.x * SUB; base_ < A.nchunks; base_ += gridDim.x * SUB) {      \
+        const int c_ = base_ + sub_;                                                       \
+        const bool cact_ = c_ < A.nchunks;                                                 \
         _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_) {                                  \
-            const int slice = c_ * kSlicesPerChunk + i_ * 4 + (threadIdx.x >> 6);          \
-            const int row = c_ * kChunk + i_ * kThreads + threadIdx.x;                      \
-            const int lane = threadIdx.x & 63;                                              \
+            const int slice = c_ * kSlicesPerChunk + i_ * 4 + (t_ >> 6);                   \
+            const int row = c_ * kChunk + i_ * kThreads + t_;                               \
+            const int lane = t_ & 63;                                                       \
             (void)slice;                                                                    \
             (void)lane;                                                                     \
-            if (row < A.n) {                                                                \
+            if (cact_ && row < A.n) {                                                       \
                 __VA_ARGS__                                                                 \
             }                                                                               \
         }                                                                                   \
@@ -126,14 +150,15 @@ __device__ __forceinline__ bool is_finite(double v) { return v - v == 0.0; }
 // Standard CG.  v[0]=r, v[1]=p, v[2]=s.  part: [0]=gamma, [1]=delta, [2]=bb.
 struct StdPhases {
     // r = b - A x0 ; p = r ; x = x0 ; partials <r,r>, <b,b>
-    static __device__ void init(const CgArgs &A, double *lds4) {
+    template <int SUB>
+    static __device__ void init(const CgArgs &A, double *lds) {
         double *r = A.v[0], *p = A.v[1];
         double ag = 0.0, ab = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sg = block256_sum(ag, lds4);                                           \
-        const double sb = block256_sum(ab, lds4);                                           \
-        if (threadIdx.x == 0) {                                                             \
+        const double sg = sub_sum<SUB>(ag, lds);                                           \
+        const double sb = sub_sum<SUB>(ab, lds);                                           \
+        if (t_ == 0 && cact_) {                                                             \
             A.part[0 * A.nchunks + c_] = sg;                                                \
             A.part[2 * A.nchunks + c_] = sb;                                                \
         }                                                                                   \
@@ -153,14 +178,15 @@ struct StdPhases {
 #undef CHUNK_END
     }
     // s = A p ; partial <p,s>
-    static __device__ void spmv_dot(const CgArgs &A, double *lds4) {
+    template <int SUB>
+    static __device__ void spmv_dot(const CgArgs &A, double *lds) {
         const double *p = A.v[1];
         double *s = A.v[2];
         double ad = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sd = block256_sum(ad, lds4);                                           \
-        if (threadIdx.x == 0) A.part[1 * A.nchunks + c_] = sd;                              \
+        const double sd = sub_sum<SUB>(ad, lds);                                           \
+        if (t_ == 0 && cact_) A.part[1 * A.nchunks + c_] = sd;                              \
         ad = 0.0;                                                                           \
     }
         CFM_FOR_MY_ROWS({
@@ -171,14 +197,15 @@ struct StdPhases {
 #undef CHUNK_END
     }
     // x += alpha p ; r -= alpha s ; partial <r,r>
-    static __device__ void update_xr(const CgArgs &A, double alpha, double *lds4) {
+    template <int SUB>
+    static __device__ void update_xr(const CgArgs &A, double alpha, double *lds) {
         double *r = A.v[0];
         const double *p = A.v[1], *s = A.v[2];
         double ag = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sg = block256_sum(ag, lds4);                                           \
-        if (threadIdx.x == 0) A.part[0 * A.nchunks + c_] = sg;                              \
+        const double sg = sub_sum<SUB>(ag, lds);                                           \
+        if (t_ == 0 && cact_) A.part[0 * A.nchunks + c_] = sg;                              \
         ag = 0.0;                                                                           \
     }
         CFM_FOR_MY_ROWS({
@@ -190,6 +217,7 @@ struct StdPhases {
 #undef CHUNK_END
     }
     // p = r + beta p
+    template <int SUB>
     static __device__ void update_p(const CgArgs &A, double beta) {
         const double *r = A.v[0];
         double *p = A.v[1];
@@ -203,12 +231,13 @@ struct StdPhases {
 // part: [0,1] = delta (buffers 0/1), [2,3] = gamma (buffers 0/1), [4] = bb.
 struct PipePhases {
     // r = b - A x0 ; x = x0 ; z = s = p = 0 ; partial <b,b>
-    static __device__ void init_r(const CgArgs &A, double *lds4) {
+    template <int SUB>
+    static __device__ void init_r(const CgArgs &A, double *lds) {
         double ab = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sb = block256_sum(ab, lds4);                                           \
-        if (threadIdx.x == 0) A.part[4 * A.nchunks + c_] = sb;                              \
+        const double sb = sub_sum<SUB>(ab, lds);                                           \
+        if (t_ == 0 && cact_) A.part[4 * A.nchunks + c_] = sb;                              \
         ab = 0.0;                                                                           \
     }
         CFM_FOR_MY_ROWS({
@@ -224,13 +253,14 @@ struct PipePhases {
 #undef CHUNK_END
     }
     // w0 = A r ; partials <r,r>, <r,w0> into buffer 0
-    static __device__ void init_w(const CgArgs &A, double *lds4) {
+    template <int SUB>
+    static __device__ void init_w(const CgArgs &A, double *lds) {
         double ad = 0.0, ag = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sd = block256_sum(ad, lds4);                                           \
-        const double sg = block256_sum(ag, lds4);                                           \
-        if (threadIdx.x == 0) {                                                             \
+        const double sd = sub_sum<SUB>(ad, lds);                                           \
+        const double sg = sub_sum<SUB>(ag, lds);                                           \
+        if (t_ == 0 && cact_) {                                                             \
             A.part[0 * A.nchunks + c_] = sd;                                                \
             A.part[2 * A.nchunks + c_] = sg;                                                \
         }                                                                                   \
@@ -247,8 +277,9 @@ struct PipePhases {
 #undef CHUNK_END
     }
     // q = A w_cur ; z,s,p,x,r,w updates ; partials for the next iteration
+    template <int SUB>
     static __device__ void iterate(const CgArgs &A, int buf, double alpha, double beta,
-                                   double *lds4) {
+                                   double *lds) {
         const double *wc = A.v[4 + buf];
         double *wn = A.v[5 - buf];
         double *r = A.v[0], *z = A.v[1], *s = A.v[2], *p = A.v[3];
@@ -256,9 +287,9 @@ struct PipePhases {
         double ad = 0.0, ag = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sd = block256_sum(ad, lds4);                                           \
-        const double sg = block256_sum(ag, lds4);                                           \
-        if (threadIdx.x == 0) {                                                             \
+        const double sd = sub_sum<SUB>(ad, lds);                                           \
+        const double sg = sub_sum<SUB>(ag, lds);                                           \
+        if (t_ == 0 && cact_) {                                                             \
             A.part[(0 + nb) * A.nchunks + c_] = sd;                                         \
             A.part[(2 + nb) * A.nchunks + c_] = sg;                                         \
         }                                                                                   \
@@ -290,14 +321,15 @@ struct PipePhases {
 // part: [0]=rr (outer), [1]=g (inner <r,r>), [2]=delta, [3]=bb.
 struct IrPhases {
     // r64 = b - A x ; rf = (float) r64 ; p = rf ; d = 0 ; partials rr, g (and bb when first)
-    static __device__ void outer(const CgArgs &A, bool first, double *lds4) {
+    template <int SUB>
+    static __device__ void outer(const CgArgs &A, bool first, double *lds) {
         double arr = 0.0, ag = 0.0, ab = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double s1 = block256_sum(arr, lds4);                                          \
-        const double s2 = block256_sum(ag, lds4);                                           \
-        const double s3 = first ? block256_sum(ab, lds4) : 0.0;                             \
-        if (threadIdx.x == 0) {                                                             \
+        const double s1 = sub_sum<SUB>(arr, lds);                                          \
+        const double s2 = sub_sum<SUB>(ag, lds);                                           \
+        const double s3 = first ? sub_sum<SUB>(ab, lds) : 0.0;                             \
+        if (t_ == 0 && cact_) {                                                             \
             A.part[0 * A.nchunks + c_] = s1;                                                \
             A.part[1 * A.nchunks + c_] = s2;                                                \
             if (first) A.part[3 * A.nchunks + c_] = s3;                                     \
@@ -323,14 +355,15 @@ struct IrPhases {
         })
 #undef CHUNK_END
     }
-    static __device__ void spmv_dot(const CgArgs &A, double *lds4) {
+    template <int SUB>
+    static __device__ void spmv_dot(const CgArgs &A, double *lds) {
         const float *p = A.f[1];
         float *s = A.f[2];
         double ad = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sd = block256_sum(ad, lds4);                                           \
-        if (threadIdx.x == 0) A.part[2 * A.nchunks + c_] = sd;                              \
+        const double sd = sub_sum<SUB>(ad, lds);                                           \
+        if (t_ == 0 && cact_) A.part[2 * A.nchunks + c_] = sd;                              \
         ad = 0.0;                                                                           \
     }
         CFM_FOR_MY_ROWS({
@@ -340,14 +373,15 @@ struct IrPhases {
         })
 #undef CHUNK_END
     }
-    static __device__ void update_dr(const CgArgs &A, float af, double *lds4) {
+    template <int SUB>
+    static __device__ void update_dr(const CgArgs &A, float af, double *lds) {
         float *rf = A.f[0], *d = A.f[3];
         const float *p = A.f[1], *s = A.f[2];
         double ag = 0.0;
 #define CHUNK_END                                                                           \
     {                                                                                       \
-        const double sg = block256_sum(ag, lds4);                                           \
-        if (threadIdx.x == 0) A.part[1 * A.nchunks + c_] = sg;                              \
+        const double sg = sub_sum<SUB>(ag, lds);                                           \
+        if (t_ == 0 && cact_) A.part[1 * A.nchunks + c_] = sg;                              \
         ag = 0.0;                                                                           \
     }
         CFM_FOR_MY_ROWS({
@@ -358,6 +392,7 @@ struct IrPhases {
         })
 #undef CHUNK_END
     }
+    template <int SUB>
     static __device__ void update_p(const CgArgs &A, float bf) {
         const float *rf = A.f[0];
         float *p = A.f[1];
@@ -365,6 +400,7 @@ struct IrPhases {
         CFM_FOR_MY_ROWS({ p[row] = fmaf(bf, p[row], rf[row]); })
 #undef CHUNK_END
     }
+    template <int SUB>
     static __device__ void update_x(const CgArgs &A) {
         const float *d = A.f[3];
 #define CHUNK_END
@@ -385,29 +421,30 @@ __device__ __forceinline__ void write_hist(const CgArgs &A, int k, double v) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && A.hist && k < A.hist_cap) A.hist[k] = v;
 }
 
-__global__ void __launch_bounds__(256) cg_standard_kernel(CgArgs A) {
-    __shared__ double lds4[4];
-    unsigned long long epoch = 0;
-    StdPhases::init(A, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) cg_standard_kernel(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    unsigned int epoch = 0;
+    StdPhases::init<SUB>(A, lds);
     if (!grid_barrier(A.sync, epoch)) return;
-    double gamma = reduce_partials(A.part + 0 * A.nchunks, A.nchunks, lds4);
-    const double bb = reduce_partials(A.part + 2 * A.nchunks, A.nchunks, lds4);
+    double gamma = reduce_partials<SUB>(A.part + 0 * A.nchunks, A.nchunks, lds);
+    const double bb = reduce_partials<SUB>(A.part + 2 * A.nchunks, A.nchunks, lds);
     const double thr = A.tol > 0.0 ? A.tol * A.tol * bb : 0.0;
     write_hist(A, 0, gamma);
     int k = 0, status = CFM_OK;
     bool converged = gamma <= thr;
     while (!converged && k < A.max_iter) {
-        StdPhases::spmv_dot(A, lds4);
+        StdPhases::spmv_dot<SUB>(A, lds);
         if (!grid_barrier(A.sync, epoch)) return;
-        const double delta = reduce_partials(A.part + 1 * A.nchunks, A.nchunks, lds4);
+        const double delta = reduce_partials<SUB>(A.part + 1 * A.nchunks, A.nchunks, lds);
         if (!(delta > 0.0) || !is_finite(delta)) {
             status = CFM_EBREAKDOWN;
             break;
         }
         const double alpha = gamma / delta;
-        StdPhases::update_xr(A, alpha, lds4);
+        StdPhases::update_xr<SUB>(A, alpha, lds);
         if (!grid_barrier(A.sync, epoch)) return;
-        const double gamma_new = reduce_partials(A.part + 0 * A.nchunks, A.nchunks, lds4);
+        const double gamma_new = reduce_partials<SUB>(A.part + 0 * A.nchunks, A.nchunks, lds);
         ++k;
         write_hist(A, k, gamma_new);
         if (gamma_new <= thr) {
@@ -415,7 +452,7 @@ __global__ void __launch_bounds__(256) cg_standard_kernel(CgArgs A) {
             break;
         }
         const double beta = gamma_new / gamma;
-        StdPhases::update_p(A, beta);
+        StdPhases::update_p<SUB>(A, beta);
         if (!grid_barrier(A.sync, epoch)) return;
         gamma = gamma_new;
     }
@@ -423,22 +460,23 @@ __global__ void __launch_bounds__(256) cg_standard_kernel(CgArgs A) {
     write_info(A, status, k, 0);
 }
 
-__global__ void __launch_bounds__(256) cg_pipelined_kernel(CgArgs A) {
-    __shared__ double lds4[4];
-    unsigned long long epoch = 0;
-    PipePhases::init_r(A, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) cg_pipelined_kernel(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    unsigned int epoch = 0;
+    PipePhases::init_r<SUB>(A, lds);
     if (!grid_barrier(A.sync, epoch)) return;
-    PipePhases::init_w(A, lds4);
+    PipePhases::init_w<SUB>(A, lds);
     if (!grid_barrier(A.sync, epoch)) return;
-    const double bb = reduce_partials(A.part + 4 * A.nchunks, A.nchunks, lds4);
+    const double bb = reduce_partials<SUB>(A.part + 4 * A.nchunks, A.nchunks, lds);
     const double thr = A.tol > 0.0 ? A.tol * A.tol * bb : 0.0;
     double alpha = 0.0, beta = 0.0, delta_prev = 0.0;
     int k = 0, status = CFM_OK;
     bool converged = false;
     for (;;) {
         const int buf = k & 1;
-        const double delta = reduce_partials(A.part + (0 + buf) * A.nchunks, A.nchunks, lds4);
-        const double gam = reduce_partials(A.part + (2 + buf) * A.nchunks, A.nchunks, lds4);
+        const double delta = reduce_partials<SUB>(A.part + (0 + buf) * A.nchunks, A.nchunks, lds);
+        const double gam = reduce_partials<SUB>(A.part + (2 + buf) * A.nchunks, A.nchunks, lds);
         write_hist(A, k, delta);
         if (delta <= thr) {
             converged = true;
@@ -458,7 +496,7 @@ __global__ void __launch_bounds__(256) cg_pipelined_kernel(CgArgs A) {
             break;
         }
         alpha = delta / denom;
-        PipePhases::iterate(A, buf, alpha, beta, lds4);
+        PipePhases::iterate<SUB>(A, buf, alpha, beta, lds);
         if (!grid_barrier(A.sync, epoch)) return;
         delta_prev = delta;
         ++k;
@@ -467,18 +505,19 @@ __global__ void __launch_bounds__(256) cg_pipelined_kernel(CgArgs A) {
     write_info(A, status, k, 0);
 }
 
-__global__ void __launch_bounds__(256) cg_ir_kernel(CgArgs A) {
-    __shared__ double lds4[4];
-    unsigned long long epoch = 0;
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) cg_ir_kernel(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    unsigned int epoch = 0;
     double thr = 0.0;
     int outer = 0, inner_total = 0, status = CFM_ENOTCONV;
     for (;;) {
         const bool first = outer == 0;
-        IrPhases::outer(A, first, lds4);
+        IrPhases::outer<SUB>(A, first, lds);
         if (!grid_barrier(A.sync, epoch)) return;
-        const double rr = reduce_partials(A.part + 0 * A.nchunks, A.nchunks, lds4);
+        const double rr = reduce_partials<SUB>(A.part + 0 * A.nchunks, A.nchunks, lds);
         if (first) {
-            const double bb = reduce_partials(A.part + 3 * A.nchunks, A.nchunks, lds4);
+            const double bb = reduce_partials<SUB>(A.part + 3 * A.nchunks, A.nchunks, lds);
             thr = A.tol * A.tol * bb;
         }
         write_hist(A, outer, rr);
@@ -487,31 +526,31 @@ __global__ void __launch_bounds__(256) cg_ir_kernel(CgArgs A) {
             break;
         }
         if (outer == A.max_outer) break;
-        double g = reduce_partials(A.part + 1 * A.nchunks, A.nchunks, lds4);
+        double g = reduce_partials<SUB>(A.part + 1 * A.nchunks, A.nchunks, lds);
         const double thr_in = A.inner_tol * A.inner_tol * g;
         int j = 0;
         bool brk = false;
         while (j < A.max_iter) {
-            IrPhases::spmv_dot(A, lds4);
+            IrPhases::spmv_dot<SUB>(A, lds);
             if (!grid_barrier(A.sync, epoch)) return;
-            const double delta = reduce_partials(A.part + 2 * A.nchunks, A.nchunks, lds4);
+            const double delta = reduce_partials<SUB>(A.part + 2 * A.nchunks, A.nchunks, lds);
             if (!(delta > 0.0) || !is_finite(delta)) {
                 brk = true;
                 break;
             }
             const float af = (float)(g / delta);
-            IrPhases::update_dr(A, af, lds4);
+            IrPhases::update_dr<SUB>(A, af, lds);
             if (!grid_barrier(A.sync, epoch)) return;
-            const double gn = reduce_partials(A.part + 1 * A.nchunks, A.nchunks, lds4);
+            const double gn = reduce_partials<SUB>(A.part + 1 * A.nchunks, A.nchunks, lds);
             ++j;
             if (gn <= thr_in) break;
             const float bf = (float)(gn / g);
-            IrPhases::update_p(A, bf);
+            IrPhases::update_p<SUB>(A, bf);
             if (!grid_barrier(A.sync, epoch)) return;
             g = gn;
         }
         inner_total += j;
-        IrPhases::update_x(A);
+        IrPhases::update_x<SUB>(A);
         if (!grid_barrier(A.sync, epoch)) return;
         ++outer;
         if (brk && j == 0) {
@@ -525,39 +564,47 @@ __global__ void __launch_bounds__(256) cg_ir_kernel(CgArgs A) {
 // ===================================================== discrete kernels ===
 // One launch per phase; scalars are reduced by a one-workgroup kernel and
 // read back by the host, which forms alpha and beta itself.
-__global__ void __launch_bounds__(256) k_std_init(CgArgs A) {
-    __shared__ double lds4[4];
-    StdPhases::init(A, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_std_init(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    StdPhases::init<SUB>(A, lds);
 }
-__global__ void __launch_bounds__(256) k_std_spmv_dot(CgArgs A) {
-    __shared__ double lds4[4];
-    StdPhases::spmv_dot(A, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_std_spmv_dot(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    StdPhases::spmv_dot<SUB>(A, lds);
 }
-__global__ void __launch_bounds__(256) k_std_update_xr(CgArgs A, double alpha) {
-    __shared__ double lds4[4];
-    StdPhases::update_xr(A, alpha, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_std_update_xr(CgArgs A, double alpha) {
+    __shared__ double lds[4 * SUB];
+    StdPhases::update_xr<SUB>(A, alpha, lds);
 }
-__global__ void __launch_bounds__(256) k_std_update_p(CgArgs A, double beta) {
-    StdPhases::update_p(A, beta);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_std_update_p(CgArgs A, double beta) {
+    StdPhases::update_p<SUB>(A, beta);
 }
-__global__ void __launch_bounds__(256) k_pipe_init_r(CgArgs A) {
-    __shared__ double lds4[4];
-    PipePhases::init_r(A, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_pipe_init_r(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    PipePhases::init_r<SUB>(A, lds);
 }
-__global__ void __launch_bounds__(256) k_pipe_init_w(CgArgs A) {
-    __shared__ double lds4[4];
-    PipePhases::init_w(A, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_pipe_init_w(CgArgs A) {
+    __shared__ double lds[4 * SUB];
+    PipePhases::init_w<SUB>(A, lds);
 }
-__global__ void __launch_bounds__(256) k_pipe_iterate(CgArgs A, int buf, double alpha, double beta) {
-    __shared__ double lds4[4];
-    PipePhases::iterate(A, buf, alpha, beta, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_pipe_iterate(CgArgs A, int buf, double alpha, double beta) {
+    __shared__ double lds[4 * SUB];
+    PipePhases::iterate<SUB>(A, buf, alpha, beta, lds);
 }
 // out[i] = reduce(part + idx[i]*nchunks), i < count
-__global__ void __launch_bounds__(256) k_reduce(CgArgs A, int i0, int i1, double *out) {
-    __shared__ double lds4[4];
-    const double a = reduce_partials(A.part + i0 * A.nchunks, A.nchunks, lds4);
+template <int SUB>
+__global__ void __launch_bounds__(256 * SUB) k_reduce(CgArgs A, int i0, int i1, double *out) {
+    __shared__ double lds[4 * SUB];
+    const double a = reduce_partials<SUB>(A.part + i0 * A.nchunks, A.nchunks, lds);
     double b = 0.0;
-    if (i1 >= 0) b = reduce_partials(A.part + i1 * A.nchunks, A.nchunks, lds4);
+    if (i1 >= 0) b = reduce_partials<SUB>(A.part + i1 * A.nchunks, A.nchunks, lds);
     if (threadIdx.x == 0) {
         out[0] = a;
         out[1] = b;
@@ -567,6 +614,48 @@ __global__ void __launch_bounds__(256) k_reduce(CgArgs A, int i0, int i1, double
 }  // namespace cfm
 
 using namespace cfm;
+
+using CgKernelFn = void (*)(CgArgs);
+
+// The kernels of one workgroup shape (SUB sub-blocks of 256 threads).
+struct CgKernels {
+    CgKernelFn standard, pipelined, ir;
+    void (*std_init)(CgArgs);
+    void (*std_spmv_dot)(CgArgs);
+    void (*std_update_xr)(CgArgs, double);
+    void (*std_update_p)(CgArgs, double);
+    void (*pipe_init_r)(CgArgs);
+    void (*pipe_init_w)(CgArgs);
+    void (*pipe_iterate)(CgArgs, int, double, double);
+    void (*reduce)(CgArgs, int, int, double *);
+    CgKernelFn persistent(int method) const {
+        return method == CFM_CG_STANDARD ? standard : method == CFM_CG_PIPELINED ? pipelined : ir;
+    }
+};
+
+template <int SUB>
+static CgKernels kernels_for() {
+    return CgKernels{cg_standard_kernel<SUB>, cg_pipelined_kernel<SUB>, cg_ir_kernel<SUB>,
+                     k_std_init<SUB>,         k_std_spmv_dot<SUB>,      k_std_update_xr<SUB>,
+                     k_std_update_p<SUB>,     k_pipe_init_r<SUB>,       k_pipe_init_w<SUB>,
+                     k_pipe_iterate<SUB>,     k_reduce<SUB>};
+}
+
+static bool kernel_table(int sub, CgKernels *out) {
+    switch (sub) {
+        case 1: *out = kernels_for<1>(); return true;
+        case 2: *out = kernels_for<2>(); return true;
+        case 3: *out = kernels_for<3>(); return true;
+        case 4: *out = kernels_for<4>(); return true;
+        default: return false;
+    }
+}
+
+template <typename... Args>
+static hipError_t launch(void (*f)(Args...), dim3 grid, dim3 block, hipStream_t s, Args... args) {
+    void *ptrs[] = {&args...};
+    return hipLaunchKernel((const void *)f, grid, block, ptrs, 0, s);
+}
 
 // ==================================================================== host
 struct cfm_cg_plan {
@@ -586,6 +675,8 @@ struct cfm_cg_plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int grid = 0;
+    int sub = 4;  // 256-thread sub-blocks per workgroup
+    CgKernels K{};
     unsigned long long spin_ticks = 0;
 };
 
@@ -621,17 +712,6 @@ static cfm_status build_sell(int n, const int *rowptr, const int *col, const dou
         }
     }
     return CFM_OK;
-}
-
-using CgKernelFn = void (*)(CgArgs);
-
-static CgKernelFn persistent_kernel(int method) {
-    switch (method) {
-        case CFM_CG_STANDARD: return cg_standard_kernel;
-        case CFM_CG_PIPELINED: return cg_pipelined_kernel;
-        case CFM_CG_IR_F32: return cg_ir_kernel;
-        default: return nullptr;
-    }
 }
 
 extern "C" void cfm_cg_plan_destroy(cfm_cg_plan *pl) {
@@ -744,14 +824,17 @@ extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, cons
     if (hipStreamCreate(&pl->stream) != hipSuccess || hipEventCreate(&pl->ev0) != hipSuccess ||
         hipEventCreate(&pl->ev1) != hipSuccess)
         return fail(CFM_EDEVICE);
+    const int bt = p->block_threads > 0 ? p->block_threads : 1024;
+    if (bt % kThreads != 0 || !kernel_table(bt / kThreads, &pl->K)) return fail(CFM_EINVAL);
+    pl->sub = bt / kThreads;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)persistent_kernel(p->method),
-                                                     kThreads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pl->K.persistent(p->method), bt, 0) !=
+            hipSuccess ||
         per_cu < 1)
         return fail(CFM_EDEVICE);
-    const int max_grid = per_cu * pl->dev.cus;
+    const int max_grid = std::min(per_cu * pl->dev.cus, kMaxGrid);
     int want = p->grid_blocks > 0 ? p->grid_blocks : max_grid;
-    pl->grid = std::max(1, std::min({want, max_grid, pl->nchunks}));
+    pl->grid = std::max(1, std::min({want, max_grid, (pl->nchunks + pl->sub - 1) / pl->sub}));
     int rate_khz = 0;
     const unsigned long long us = p->spin_limit_us > 0 ? p->spin_limit_us : 10000000ull;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, pl->dev.device) != hipSuccess ||
@@ -810,11 +893,11 @@ static CgArgs make_args(cfm_cg_plan *pl) {
 // Host-driven schedule of the same phases (reference "Baseline Discrete").
 static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &hist, int *iters_out,
                                int *status_out) {
-    const dim3 grid(pl->grid), block(kThreads);
+    const dim3 grid(pl->grid), block(kThreads * pl->sub);
+    const CgKernels &K = pl->K;
     hipStream_t s = pl->stream;
     auto fetch = [&](int i0, int i1, double *a, double *b) -> cfm_status {
-        hipLaunchKernelGGL(k_reduce, dim3(1), block, 0, s, A, i0, i1, pl->scal);
-        CFM_HIP(hipGetLastError());
+        CFM_HIP(launch(K.reduce, dim3(1), block, s, A, i0, i1, pl->scal));
         CFM_HIP(hipMemcpyAsync(pl->scal_h, pl->scal, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
         CFM_HIP(hipStreamSynchronize(s));
         *a = pl->scal_h[0];
@@ -827,14 +910,14 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
     bool converged = false;
     cfm_status st;
     if (pl->p.method == CFM_CG_STANDARD) {
-        hipLaunchKernelGGL(k_std_init, grid, block, 0, s, A);
+        CFM_HIP(launch(K.std_init, grid, block, s, A));
         double gamma, bb;
         if ((st = fetch(0, 2, &gamma, &bb)) != CFM_OK) return st;
         const double thr = tol > 0.0 ? tol * tol * bb : 0.0;
         hist.push_back(gamma);
         converged = gamma <= thr;
         while (!converged && k < max_iter) {
-            hipLaunchKernelGGL(k_std_spmv_dot, grid, block, 0, s, A);
+            CFM_HIP(launch(K.std_spmv_dot, grid, block, s, A));
             double delta;
             if ((st = fetch(1, -1, &delta, nullptr)) != CFM_OK) return st;
             if (!(delta > 0.0) || !std::isfinite(delta)) {
@@ -842,7 +925,7 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
                 break;
             }
             const double alpha = gamma / delta;
-            hipLaunchKernelGGL(k_std_update_xr, grid, block, 0, s, A, alpha);
+            CFM_HIP(launch(K.std_update_xr, grid, block, s, A, alpha));
             double gamma_new;
             if ((st = fetch(0, -1, &gamma_new, nullptr)) != CFM_OK) return st;
             ++k;
@@ -852,12 +935,12 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
                 break;
             }
             const double beta = gamma_new / gamma;
-            hipLaunchKernelGGL(k_std_update_p, grid, block, 0, s, A, beta);
+            CFM_HIP(launch(K.std_update_p, grid, block, s, A, beta));
             gamma = gamma_new;
         }
     } else {
-        hipLaunchKernelGGL(k_pipe_init_r, grid, block, 0, s, A);
-        hipLaunchKernelGGL(k_pipe_init_w, grid, block, 0, s, A);
+        CFM_HIP(launch(K.pipe_init_r, grid, block, s, A));
+        CFM_HIP(launch(K.pipe_init_w, grid, block, s, A));
         double bb, dummy;
         if ((st = fetch(4, -1, &bb, nullptr)) != CFM_OK) return st;
         (void)dummy;
@@ -886,7 +969,7 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
                 break;
             }
             alpha = delta / denom;
-            hipLaunchKernelGGL(k_pipe_iterate, grid, block, 0, s, A, buf, alpha, beta);
+            CFM_HIP(launch(K.pipe_iterate, grid, block, s, A, buf, alpha, beta));
             delta_prev = delta;
             ++k;
         }
@@ -917,12 +1000,12 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
     int status = CFM_OK, k = 0, outer = 0;
     float ms = 0.f;
     if (pl->p.mode == CFM_CG_PERSISTENT) {
-        hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, pl->stream, pl->sync, pl->spin_ticks);
+        hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
         CFM_HIP(hipGetLastError());
         CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
         void *args[] = {&A};
-        CFM_HIP(hipLaunchCooperativeKernel((const void *)persistent_kernel(pl->p.method), dim3(pl->grid),
-                                           dim3(kThreads), args, 0, pl->stream));
+        CFM_HIP(hipLaunchCooperativeKernel((const void *)pl->K.persistent(pl->p.method), dim3(pl->grid),
+                                           dim3(kThreads * pl->sub), args, 0, pl->stream));
         CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
         CFM_HIP(hipEventSynchronize(pl->ev1));
         CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
@@ -954,7 +1037,7 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
         t->kernel_ms = ms;
         t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
         t->grid_blocks = pl->grid;
-        t->block_threads = kThreads;
+        t->block_threads = kThreads * pl->sub;
         t->iterations = k;
         t->outer_iterations = outer;
     }

@@ -35,12 +35,14 @@
 
 namespace cfm {
 
-__global__ void reset_sync_kernel(GridSync *s, unsigned long long limit_ticks) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        s->count = 0;
+__global__ void reset_sync_kernel(GridSync *s, unsigned long long limit_ticks, int grid) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < kGoCopies * 32) s->go[t] = 0;
+    if (t == 0) {
         s->abort = 0;
         s->spin_limit_ticks = limit_ticks;
     }
+    for (int b = t; b < grid; b += gridDim.x * blockDim.x) s->arrive[b] = 0;
 }
 
 struct JacobiArgs {
@@ -59,16 +61,32 @@ struct JacobiArgs {
     GridSync *sync;
 };
 
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
 template <typename T>
 struct Vec16;
 template <>
 struct Vec16<double> {
-    using type = double2;
+    using type = v2f64;
 };
 template <>
 struct Vec16<float> {
-    using type = float4;
+    using type = v4f32;
 };
+
+// Interior stores of the new field: it is not read again before the next
+// sweep, by which time it has left the caches, so it may bypass them.
+#ifndef CFM_JACOBI_NT_STORE
+#define CFM_JACOBI_NT_STORE 1
+#endif
+template <typename V>
+__device__ __forceinline__ void store16(V *p, V v) {
+#if CFM_JACOBI_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 
 template <typename T>
 __device__ __forceinline__ T stencil_coeff3d() {
@@ -81,11 +99,10 @@ __device__ __forceinline__ void load_row(const T *__restrict__ p, int x, int nx,
                                          T (&v)[VX]) {
     if (ALIGNED) {
         if (row_ok && x + VX <= nx) {
-            typename Vec16<T>::type t = *reinterpret_cast<const typename Vec16<T>::type *>(p + x);
+            const typename Vec16<T>::type t = *reinterpret_cast<const typename Vec16<T>::type *>(p + x);
             static_assert(sizeof(t) == VX * sizeof(T), "vector width");
-            const T *tp = reinterpret_cast<const T *>(&t);
 #pragma unroll
-            for (int e = 0; e < VX; ++e) v[e] = tp[e];
+            for (int e = 0; e < VX; ++e) v[e] = t[e];
             return;
         }
     }
@@ -199,10 +216,9 @@ __device__ __forceinline__ void sweep3d_tile(const T *__restrict__ src, T *__res
                 T *drow = dz + (size_t)y * nx;
                 if (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2) {
                     typename Vec16<T>::type t;
-                    T *tp = reinterpret_cast<T *>(&t);
 #pragma unroll
-                    for (int e = 0; e < VX; ++e) tp[e] = out[e];
-                    *reinterpret_cast<typename Vec16<T>::type *>(drow + x) = t;
+                    for (int e = 0; e < VX; ++e) t[e] = out[e];
+                    store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x), t);
                 } else {
 #pragma unroll
                     for (int e = 0; e < VX; ++e)
@@ -237,14 +253,14 @@ __device__ __forceinline__ void sweep3d_tile(const T *__restrict__ src, T *__res
     }
 }
 
-template <typename T, int RY, bool ALIGNED>
-__global__ void __launch_bounds__(256) jacobi3d_kernel(JacobiArgs A) {
-    __shared__ double lds4[4];
+template <typename T, int RY, bool ALIGNED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) jacobi3d_kernel(JacobiArgs A) {
+    __shared__ double lds[BLOCK / 64];
     const int wave_id = xcd_wave_id();
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     const int ntiles = A.ntx * A.nty;
     const int nunits = ntiles * A.chunks;
-    unsigned long long epoch = 0;
+    unsigned int epoch = 0;
     for (int k = 0; k < A.niter; ++k) {
         const int it = A.it0 + k;
         const bool want = A.norm_every > 0 && (it % A.norm_every) == 0;
@@ -260,7 +276,237 @@ __global__ void __launch_bounds__(256) jacobi3d_kernel(JacobiArgs A) {
                 sweep3d_tile<T, RY, ALIGNED>(src, dst, A, tx, ty, zc, want, nacc);
             }
             if (want) {
-                const double s = block256_sum(nacc, lds4);
+                const double s = block_sum_waves<BLOCK / 64>(nacc, lds);
+                if (threadIdx.x == 0) A.norm_part[(size_t)(it / A.norm_every) * gridDim.x + blockIdx.x] = s;
+            }
+        }
+        if (k + 1 < A.niter) {
+            if (!grid_barrier(A.sync, epoch)) return;
+        }
+    }
+}
+
+// ------------------------------------------------------- 3D, workgroup tile --
+// A workgroup of WX*WY wavefronts owns a tile of TW = WX*64*VX columns by
+// TH = WY*RY rows and streams it through a z-chunk.  Each wavefront keeps
+// only its own RY rows of planes z-1, z, z+1 in registers; the rows just
+// outside its own (the neighbour wavefront's first/last row, or the tile's
+// halo row loaded from global memory by the edge wavefronts) and the x-edge
+// values at its lanes 0/63 are exchanged through LDS, double-buffered by the
+// plane parity, one __syncthreads per plane.  Global loads per plane: the
+// tile's own rows once, plus two halo rows and two edge columns per tile.
+template <typename T, int RY, int WX, int WY>
+struct WgTile {
+    static constexpr int VX = 16 / sizeof(T);
+    static constexpr int SEG = 64 * VX;
+    static constexpr int TW = SEG * WX;
+    static constexpr int TH = RY * WY;
+    static constexpr int BLOCK = 64 * WX * WY;
+    // per plane parity
+    struct Plane {
+        T first[WY + 1][TW];    // first row of wave k (k < WY); [WY] = halo row y0 + TH
+        T last[WY + 1][TW];     // [0] = halo row y0 - 1; [k] = last row of wave k-1
+        T edge_l[WY][WX + 1][RY];  // [k] = first column of wave k; [WX] = column x0 + TW
+        T edge_r[WY][WX + 1][RY];  // [0] = column x0 - 1; [k] = last column of wave k-1
+    };
+};
+
+template <typename T, int RY, int WX, int WY, bool ALIGNED>
+__device__ __forceinline__ void sweep3d_wgtile(const T *__restrict__ src, T *__restrict__ dst,
+                                               const JacobiArgs &A, int tx, int ty, int zc,
+                                               bool want_norm, double &nacc,
+                                               typename WgTile<T, RY, WX, WY>::Plane *lds) {
+    using Tl = WgTile<T, RY, WX, WY>;
+    constexpr int VX = Tl::VX;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wx = wave % WX, wy = wave / WX;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const size_t plane = (size_t)nx * ny;
+    const int xl = wx * Tl::SEG + lane * VX;  // column inside the tile
+    const int x0 = tx * Tl::TW;
+    const int x = x0 + xl;
+    const int ty0 = ty * Tl::TH;
+    const int y0 = ty0 + wy * RY;
+    const int zb = 1 + zc * A.chunk_len;
+    const int ze = min(zb + A.chunk_len, nz - 1);
+    if (zb >= ze) return;  // uniform over the workgroup
+    const T c = stencil_coeff3d<T>();
+    const bool periodic = A.bc == CFM_BC_PERIODIC;
+    // global halo duties of this wavefront
+    const bool top = wy == 0, bot = wy == WY - 1;
+    const int yt = ty0 - 1, yb = ty0 + Tl::TH;
+    const bool ld_top = top && yt >= 0;
+    const bool ld_bot = bot && yb < ny;
+    const bool ld_el = wx == 0 && lane == 0 && x0 - 1 >= 0;             // column x0-1
+    const bool ld_er = wx == WX - 1 && lane == 63 && x0 + Tl::TW < nx;  // column x0+TW
+
+    T prev[RY][VX], cur[RY][VX], nxt[RY][VX];
+    T hcur[VX] = {}, hnxt[VX] = {};  // this wave's halo row (top or bottom wave only)
+    T ecur[RY] = {}, enxt[RY] = {};  // this lane's edge column (x-edge lanes only)
+
+    auto load_plane = [&](int z, T (&rows)[RY][VX], T (&h)[VX], T (&e)[RY]) {
+        const T *pz = src + (size_t)z * plane;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int y = y0 + r;
+            load_row<T, VX, ALIGNED>(pz + (size_t)(y < ny ? y : 0) * nx, x, nx, y < ny, rows[r]);
+        }
+        if (top || bot) {
+            const bool ok = top ? ld_top : ld_bot;
+            const int y = top ? yt : yb;
+            load_row<T, VX, ALIGNED>(pz + (size_t)(ok ? y : 0) * nx, x, nx, ok, h);
+        }
+        if (ld_el || ld_er) {
+            const int xe = ld_el ? x0 - 1 : x0 + Tl::TW;
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                const int y = y0 + r;
+                e[r] = y < ny ? pz[(size_t)y * nx + xe] : T(0);
+            }
+        }
+    };
+
+    {
+        T hd[VX], ed[RY];
+        load_plane(zb - 1, prev, hd, ed);
+        load_plane(zb, cur, hcur, ecur);
+    }
+    for (int z = zb; z < ze; ++z) {
+        load_plane(z + 1, nxt, hnxt, enxt);  // z+1 <= nz-1
+        typename Tl::Plane &P = lds[z & 1];
+        // publish this wave's boundary rows / columns of plane z
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            P.first[wy][xl + e] = cur[0][e];
+            P.last[wy + 1][xl + e] = cur[RY - 1][e];
+        }
+        if (top)
+#pragma unroll
+            for (int e = 0; e < VX; ++e) P.last[0][xl + e] = hcur[e];
+        if (bot)
+#pragma unroll
+            for (int e = 0; e < VX; ++e) P.first[WY][xl + e] = hcur[e];
+        if (lane == 0)
+#pragma unroll
+            for (int r = 0; r < RY; ++r) P.edge_l[wy][wx][r] = cur[r][0];
+        if (lane == 63)
+#pragma unroll
+            for (int r = 0; r < RY; ++r) P.edge_r[wy][wx + 1][r] = cur[r][VX - 1];
+        if (wx == 0 && lane == 0)
+#pragma unroll
+            for (int r = 0; r < RY; ++r) P.edge_r[wy][0][r] = ecur[r];
+        if (wx == WX - 1 && lane == 63)
+#pragma unroll
+            for (int r = 0; r < RY; ++r) P.edge_l[wy][WX][r] = ecur[r];
+        __syncthreads();
+
+        T *dz = dst + (size_t)z * plane;
+        T *dcopy = nullptr;
+        if (periodic) {
+            if (z == nz - 2) dcopy = dst;
+            else if (z == 1) dcopy = dst + (size_t)(nz - 1) * plane;
+        }
+        T *dcopy2 = (periodic && z == 1 && nz - 2 == 1) ? dst + (size_t)(nz - 1) * plane : nullptr;
+
+        T north[VX], south[VX];
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            south[e] = P.last[wy][xl + e];
+            north[e] = P.first[wy + 1][xl + e];
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int y = y0 + r;
+            const T *cr = cur[r];
+            T west = __shfl_up(cr[VX - 1], 1, 64);
+            T east = __shfl_down(cr[0], 1, 64);
+            if (lane == 0) west = P.edge_r[wy][wx][r];
+            if (lane == 63) east = P.edge_l[wy][wx + 1][r];
+            T out[VX];
+            bool inter[VX];
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const T E = (e < VX - 1) ? cr[e + 1] : east;
+                const T W = (e > 0) ? cr[e - 1] : west;
+                const T N = (r + 1 < RY) ? cur[r + 1][e] : north[e];
+                const T S = (r > 0) ? cur[r - 1][e] : south[e];
+                const T v = c * (E + W + N + S + nxt[r][e] + prev[r][e]);
+                const int xx = x + e;
+                inter[e] = xx >= 1 && xx <= nx - 2 && y >= 1 && y <= ny - 2;
+                out[e] = inter[e] ? v : cr[e];
+                if (want_norm && inter[e]) {
+                    const double d = (double)v - (double)cr[e];
+                    nacc += d * d;
+                }
+            }
+            if (y >= 1 && y <= ny - 2) {
+                T *drow = dz + (size_t)y * nx;
+                if (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2) {
+                    typename Vec16<T>::type t;
+#pragma unroll
+                    for (int e = 0; e < VX; ++e) t[e] = out[e];
+                    store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x), t);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VX; ++e)
+                        if (inter[e]) drow[x + e] = out[e];
+                }
+            }
+            if (dcopy && y < ny) {
+                T *crow = dcopy + (size_t)y * nx;
+#pragma unroll
+                for (int e = 0; e < VX; ++e)
+                    if (x + e < nx) crow[x + e] = out[e];
+                if (dcopy2) {
+                    T *crow2 = dcopy2 + (size_t)y * nx;
+#pragma unroll
+                    for (int e = 0; e < VX; ++e)
+                        if (x + e < nx) crow2[x + e] = out[e];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                prev[r][e] = cur[r][e];
+                cur[r][e] = nxt[r][e];
+            }
+#pragma unroll
+        for (int e = 0; e < VX; ++e) hcur[e] = hnxt[e];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) ecur[r] = enxt[r];
+    }
+    __syncthreads();  // the LDS planes are reused by the next unit
+}
+
+template <typename T, int RY, int WX, int WY, bool ALIGNED>
+__global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A) {
+    using Tl = WgTile<T, RY, WX, WY>;
+    __shared__ typename Tl::Plane planes[2];
+    __shared__ double lds[Tl::BLOCK / 64];
+    const int g = gridDim.x, b = blockIdx.x;
+    const int lb = (g & 7) == 0 ? (b & 7) * (g >> 3) + (b >> 3) : b;  // XCD-contiguous order
+    const int ntiles = A.ntx * A.nty;
+    const int nunits = ntiles * A.chunks;
+    unsigned int epoch = 0;
+    for (int k = 0; k < A.niter; ++k) {
+        const int it = A.it0 + k;
+        const bool want = A.norm_every > 0 && (it % A.norm_every) == 0;
+        if (!A.no_compute) {
+            const T *src = static_cast<const T *>((it & 1) ? A.buf1 : A.buf0);
+            T *dst = static_cast<T *>((it & 1) ? A.buf0 : A.buf1);
+            double nacc = 0.0;
+            for (int u = lb; u < nunits; u += g) {
+                const int zc = u / ntiles;
+                const int rem = u - zc * ntiles;
+                const int ty = rem / A.ntx;
+                const int tx = rem - ty * A.ntx;
+                sweep3d_wgtile<T, RY, WX, WY, ALIGNED>(src, dst, A, tx, ty, zc, want, nacc, planes);
+            }
+            if (want) {
+                const double s = block_sum_waves<Tl::BLOCK / 64>(nacc, lds);
                 if (threadIdx.x == 0) A.norm_part[(size_t)(it / A.norm_every) * gridDim.x + blockIdx.x] = s;
             }
         }
@@ -337,10 +583,9 @@ __device__ __forceinline__ void sweep2d_tile(const T *__restrict__ src, T *__res
                 T *drow = dst + (size_t)yy * nx;
                 if (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2) {
                     typename Vec16<T>::type t;
-                    T *tp = reinterpret_cast<T *>(&t);
 #pragma unroll
-                    for (int e = 0; e < VX; ++e) tp[e] = out[e];
-                    *reinterpret_cast<typename Vec16<T>::type *>(drow + x) = t;
+                    for (int e = 0; e < VX; ++e) t[e] = out[e];
+                    store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x), t);
                 } else {
 #pragma unroll
                     for (int e = 0; e < VX; ++e)
@@ -373,13 +618,13 @@ __device__ __forceinline__ void sweep2d_tile(const T *__restrict__ src, T *__res
     }
 }
 
-template <typename T, int RB, bool ALIGNED>
-__global__ void __launch_bounds__(256) jacobi2d_kernel(JacobiArgs A) {
-    __shared__ double lds4[4];
+template <typename T, int RB, bool ALIGNED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) jacobi2d_kernel(JacobiArgs A) {
+    __shared__ double lds[BLOCK / 64];
     const int wave_id = xcd_wave_id();
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     const int nunits = A.ntx * A.chunks;
-    unsigned long long epoch = 0;
+    unsigned int epoch = 0;
     for (int k = 0; k < A.niter; ++k) {
         const int it = A.it0 + k;
         const bool want = A.norm_every > 0 && (it % A.norm_every) == 0;
@@ -393,7 +638,7 @@ __global__ void __launch_bounds__(256) jacobi2d_kernel(JacobiArgs A) {
                 sweep2d_tile<T, RB, ALIGNED>(src, dst, A, tx, yc, want, nacc);
             }
             if (want) {
-                const double s = block256_sum(nacc, lds4);
+                const double s = block_sum_waves<BLOCK / 64>(nacc, lds);
                 if (threadIdx.x == 0) A.norm_part[(size_t)(it / A.norm_every) * gridDim.x + blockIdx.x] = s;
             }
         }
@@ -406,27 +651,51 @@ __global__ void __launch_bounds__(256) jacobi2d_kernel(JacobiArgs A) {
 // ------------------------------------------------------------- dispatch ---
 using KernelFn = void (*)(JacobiArgs);
 
-template <typename T>
-static KernelFn pick_kernel(bool is3d, int rows, bool aligned) {
-#define CFM_J3(R) (aligned ? (KernelFn)jacobi3d_kernel<T, R, true> : (KernelFn)jacobi3d_kernel<T, R, false>)
-#define CFM_J2(R) (aligned ? (KernelFn)jacobi2d_kernel<T, R, true> : (KernelFn)jacobi2d_kernel<T, R, false>)
+// Default workgroup size per tile shape: one workgroup fills a CU (fewer
+// workgroups make the device-wide barrier cheaper).  3D: WX = 4 wavefronts
+// across x and block/256 wavefronts down y share halos through LDS.
+static int default_block(bool is3d, int rows) {
+    if (is3d) return rows == 1 ? 256 : rows == 2 ? 1024 : rows == 4 ? 768 : 512;
+    return rows == 4 ? 1024 : rows == 8 ? 768 : 512;
+}
+
+template <typename T, bool AL>
+static KernelFn pick_kernel_t(bool is3d, int rows, int block) {
+#define CFM_J3(R, B) (KernelFn)jacobi3d_kernel<T, R, AL, B>
+#define CFM_W3(R, WY) (KernelFn)jacobi3d_wg_kernel<T, R, 4, WY, AL>
+#define CFM_J2(R, B) (KernelFn)jacobi2d_kernel<T, R, AL, B>
     if (is3d) {
-        switch (rows) {
-            case 1: return CFM_J3(1);
-            case 2: return CFM_J3(2);
-            case 4: return CFM_J3(4);
-            case 8: return CFM_J3(8);
-            default: return nullptr;
+        if (block == 256) {  // one independent tile per wavefront
+            switch (rows) {
+                case 1: return CFM_J3(1, 256);
+                case 2: return CFM_J3(2, 256);
+                case 4: return CFM_J3(4, 256);
+                case 8: return CFM_J3(8, 256);
+                default: return nullptr;
+            }
         }
+        if (rows == 2 && block == 1024) return CFM_W3(2, 4);
+        if (rows == 4 && block == 1024) return CFM_W3(4, 4);
+        if (rows == 4 && block == 768) return CFM_W3(4, 3);
+        if (rows == 4 && block == 512) return CFM_W3(4, 2);
+        if (rows == 8 && block == 512) return CFM_W3(8, 2);
+        return nullptr;
     }
+    const bool wide = block != 256;
     switch (rows) {
-        case 4: return CFM_J2(4);
-        case 8: return CFM_J2(8);
-        case 16: return CFM_J2(16);
+        case 4: return wide ? CFM_J2(4, 1024) : CFM_J2(4, 256);
+        case 8: return wide ? CFM_J2(8, 768) : CFM_J2(8, 256);
+        case 16: return wide ? CFM_J2(16, 512) : CFM_J2(16, 256);
         default: return nullptr;
     }
 #undef CFM_J3
+#undef CFM_W3
 #undef CFM_J2
+}
+
+template <typename T>
+static KernelFn pick_kernel(bool is3d, int rows, bool aligned, int block) {
+    return aligned ? pick_kernel_t<T, true>(is3d, rows, block) : pick_kernel_t<T, false>(is3d, rows, block);
 }
 
 cfm_status current_device_props(DeviceProps *out) {
@@ -459,7 +728,7 @@ struct cfm_jacobi_plan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     KernelFn kernel = nullptr;
     int grid = 0;
-    int chunks = 0, chunk_len = 0, ntx = 0, nty = 0, rows = 0;
+    int chunks = 0, chunk_len = 0, ntx = 0, nty = 0, rows = 0, block = 256;
     unsigned long long spin_ticks = 0;
 };
 
@@ -498,16 +767,22 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     pl->elems = (size_t)p->nx * p->ny * (size_t)p->nz;
     const int vx = 16 / (int)pl->elem_size;
     const bool aligned = ((size_t)p->nx * pl->elem_size) % 16 == 0;
-    pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? 4 : 8);
-    pl->kernel = f64 ? pick_kernel<double>(is3d, pl->rows, aligned)
-                     : pick_kernel<float>(is3d, pl->rows, aligned);
+    // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
+    pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
+    pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows);
+    if (!is3d && pl->block != 256 && pl->block != default_block(false, pl->rows)) {
+        delete pl;
+        return CFM_EINVAL;
+    }
+    pl->kernel = f64 ? pick_kernel<double>(is3d, pl->rows, aligned, pl->block)
+                     : pick_kernel<float>(is3d, pl->rows, aligned, pl->block);
     if (!pl->kernel) {
         delete pl;
         return CFM_EINVAL;
     }
-    // grid: all co-resident workgroups of 256 threads
+    // grid: all co-resident workgroups
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pl->kernel, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pl->kernel, pl->block, 0) !=
             hipSuccess ||
         per_cu < 1) {
         delete pl;
@@ -515,18 +790,20 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     }
     const int max_grid = per_cu * pl->dev.cus;
     pl->grid = p->grid_blocks > 0 ? std::min(p->grid_blocks, max_grid) : max_grid;
-    const int nwaves = pl->grid * 4;
-    pl->ntx = (p->nx + 64 * vx - 1) / (64 * vx);
+    // work units: one per wavefront (per-wave kernels) or per workgroup (3D tile kernel)
+    const bool wg_tile = is3d && pl->block != 256;
+    const int nworkers = wg_tile ? pl->grid : pl->grid * (pl->block / 64);
+    const int tile_w = wg_tile ? 64 * vx * 4 : 64 * vx;
+    const int tile_h = wg_tile ? pl->rows * (pl->block / 256) : pl->rows;
+    pl->ntx = (p->nx + tile_w - 1) / tile_w;
     const int interior = is3d ? p->nz - 2 : p->ny - 2;
-    if (is3d) {
-        pl->nty = (p->ny + pl->rows - 1) / pl->rows;
-    } else {
-        pl->nty = 1;
-    }
+    pl->nty = is3d ? (p->ny + tile_h - 1) / tile_h : 1;
     const int tiles = pl->ntx * pl->nty;
     int chunks = p->chunks;
     if (chunks <= 0) {
-        chunks = std::max(1, nwaves / tiles);
+        // 3D tiles of 2 rows: 4 z-chunks per workgroup balance best; else one
+        const int per_worker = (wg_tile && pl->rows <= 2) ? 4 : 1;
+        chunks = std::max(1, per_worker * nworkers / tiles);
         const int min_len = is3d ? 16 : 64;
         chunks = std::min(chunks, std::max(1, interior / min_len));
     }
@@ -591,7 +868,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
     A.norm_part = pl->norm_part;
     A.sync = pl->sync;
     A.no_compute = p.variant == CFM_JACOBI_NO_COMPUTE;
-    hipLaunchKernelGGL(reset_sync_kernel, dim3(1), dim3(64), 0, pl->stream, pl->sync, pl->spin_ticks);
+    hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
     CFM_HIP(hipGetLastError());
     CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
     if (p.niter > 0) {
@@ -599,14 +876,14 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
             for (int k = 0; k < p.niter; ++k) {
                 A.niter = 1;
                 A.it0 = k;
-                hipLaunchKernelGGL(pl->kernel, dim3(pl->grid), dim3(256), 0, pl->stream, A);
+                hipLaunchKernelGGL(pl->kernel, dim3(pl->grid), dim3(pl->block), 0, pl->stream, A);
                 CFM_HIP(hipGetLastError());
             }
         } else {
             A.niter = p.niter;
             A.it0 = 0;
             void *args[] = {&A};
-            CFM_HIP(hipLaunchCooperativeKernel((const void *)pl->kernel, dim3(pl->grid), dim3(256),
+            CFM_HIP(hipLaunchCooperativeKernel((const void *)pl->kernel, dim3(pl->grid), dim3(pl->block),
                                                args, 0, pl->stream));
         }
     }
@@ -635,7 +912,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
         t->kernel_ms = ms;
         t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
         t->grid_blocks = pl->grid;
-        t->block_threads = 256;
+        t->block_threads = pl->block;
         t->iterations = p.niter;
         t->outer_iterations = 0;
     }

@@ -77,6 +77,7 @@ typedef struct {
     int grid_blocks;  /* persistent grid size; 0 = auto (all co-resident workgroups) */
     int chunks;       /* chunks along the streamed axis; 0 = auto */
     int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
+    int block_threads;/* 0 = one workgroup per CU (size set by the tile shape), or 256 */
 } cfm_jacobi_params;
 
 typedef struct {
@@ -99,6 +100,11 @@ cfm_status cfm_device_count(int *count);
 cfm_status cfm_device_info(int dev, char *name, int name_cap, int *compute_units,
                            size_t *mem_bytes);
 cfm_status cfm_device_synchronize(void);
+/* Streaming copy of `bytes` (two buffers) timed over `reps` launches: the
+ * practical HBM ceiling (copy_gbs counts read + write bytes) and the
+ * calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.  A utility
+ * with no counterpart in the reference. */
+cfm_status cfm_bandwidth_probe(size_t bytes, int reps, double *copy_gbs, double *best_ms);
 
 /* ------------------------------------------------------------- Jacobi --- */
 /* Replaces the allocate/initialise/launch/swap part of
@@ -142,6 +148,7 @@ typedef struct {
     double inner_tol;    /* IR: inner relative tolerance (0 = 1e-4) */
     int grid_blocks;     /* 0 = auto */
     int spin_limit_us;   /* device barrier watchdog; 0 = default (10 s) */
+    int block_threads;   /* 256, 512, 768 or 1024 (256-row sub-blocks per workgroup); 0 = auto */
 } cfm_cg_params;
 
 /* Replaces SingleStreamStandard::init / SingleStreamPipelined::init and

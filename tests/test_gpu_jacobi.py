@@ -59,6 +59,17 @@ def test_tile_shapes_3d(gpu, orc, rows, grid):
     _check(gpu, orc, init, 5, 0, norm_every=1, tile_rows=rows, grid_blocks=grid)
 
 
+@pytest.mark.parametrize("rows,block", [(1, 256), (2, 256), (4, 256), (8, 256), (2, 1024), (4, 512),
+                                        (4, 768), (4, 1024), (8, 512)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(13, 37, 150), (9, 70, 1100)])
+def test_block_shapes_3d(gpu, orc, rows, block, dtype, shape):
+    init = _rand(shape, dtype, rows * 7 + block)
+    for bc in (0, 1):
+        _check(gpu, orc, init, 4, bc, norm_every=2, tile_rows=rows, block_threads=block)
+    _check(gpu, orc, init, 3, 1, tile_rows=rows, block_threads=block, grid_blocks=2, chunks=3)
+
+
 @pytest.mark.parametrize("rows", [4, 8, 16])
 @pytest.mark.parametrize("grid", [0, 1, 3])
 @pytest.mark.parametrize("chunks", [0, 1, 7])

@@ -35,6 +35,8 @@ EXPORTS = (
     "cfm_jacobi_plan_field", "cfm_jacobi_plan_destroy", "cfm_jacobi_run", "cfm_jacobi_init_field",
     "cfm_cg_plan_create", "cfm_cg_plan_set_rhs", "cfm_cg_plan_set_x0", "cfm_cg_plan_solve",
     "cfm_cg_plan_get_x", "cfm_cg_plan_destroy", "cfm_cg_solve", "cfm_gen_poisson3d", "cfm_gen_uniform",
+    "cfm_gen_varcoef3d", "cfm_cg_batch_plan_create", "cfm_cg_batch_set_rhs", "cfm_cg_batch_set_x0",
+    "cfm_cg_batch_solve", "cfm_cg_batch_get_x", "cfm_cg_batch_tune", "cfm_cg_batch_plan_destroy",
 )
 
 
@@ -105,6 +107,18 @@ def _load():
     L.cfm_gen_poisson3d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp]
     L.cfm_gen_poisson3d.restype = ctypes.c_int64
     L.cfm_gen_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _dp, ctypes.c_int64]
+    L.cfm_gen_varcoef3d.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp,
+                                    _dp]
+    L.cfm_gen_varcoef3d.restype = ctypes.c_int64
+    L.cfm_cg_batch_plan_create.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.POINTER(CgParams),
+                                           ctypes.POINTER(_vp)]
+    L.cfm_cg_batch_set_rhs.argtypes = [_vp, _dp]
+    L.cfm_cg_batch_set_x0.argtypes = [_vp, _dp]
+    L.cfm_cg_batch_solve.argtypes = [_vp, _ip, _ip, _dp, ctypes.c_int, ctypes.POINTER(Timing)]
+    L.cfm_cg_batch_get_x.argtypes = [_vp, _dp]
+    L.cfm_cg_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, _ip, _ip]
+    L.cfm_cg_batch_plan_destroy.argtypes = [_vp]
+    L.cfm_cg_batch_plan_destroy.restype = None
     return L
 
 
@@ -335,3 +349,110 @@ def cg(rowptr, col, val, b, x0=None, index_base=0, **kw):
         st, hist, it, t = plan.solve()
         x = plan.x()
     return st, x, hist, it, t
+
+
+# ----------------------------------------------------------------- batched --
+
+def varcoef3d(seed, system_id, lmin, lmax):
+    """Variable-coefficient system: returns (rowptr, col, val, f, dims)."""
+    dims = np.zeros(3, dtype=np.int32)
+    nnz = lib.cfm_gen_varcoef3d(seed, system_id, lmin, lmax, dims.ctypes.data_as(_ip), None, None, None, None)
+    if nnz < 0:
+        raise ValueError("bad generator arguments")
+    n = int(dims.prod())
+    rowptr = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    f = np.empty(n, dtype=np.float64)
+    lib.cfm_gen_varcoef3d(seed, system_id, lmin, lmax, dims.ctypes.data_as(_ip), rowptr.ctypes.data_as(_ip),
+                          col.ctypes.data_as(_ip), val.ctypes.data_as(_dp), f.ctypes.data_as(_dp))
+    return rowptr, col, val, f, tuple(int(d) for d in dims)
+
+
+def block_diag(systems):
+    """Concatenate [(rowptr, col, val), ...] into one block-diagonal CSR.
+    Returns (sys_row_ptr int64, rowptr, col, val)."""
+    n = [s[0].size - 1 for s in systems]
+    sys_row_ptr = np.zeros(len(systems) + 1, dtype=np.int64)
+    sys_row_ptr[1:] = np.cumsum(n)
+    nnz_off = np.zeros(len(systems) + 1, dtype=np.int64)
+    nnz_off[1:] = np.cumsum([s[1].size for s in systems])
+    rowptr = np.empty(sys_row_ptr[-1] + 1, dtype=np.int32)
+    col = np.empty(nnz_off[-1], dtype=np.int32)
+    val = np.empty(nnz_off[-1], dtype=np.float64)
+    for i, (rp, c, v) in enumerate(systems):
+        r0, e0 = sys_row_ptr[i], nnz_off[i]
+        rowptr[r0:r0 + n[i]] = rp[:-1] + e0
+        col[e0:e0 + c.size] = c + r0
+        val[e0:e0 + v.size] = v
+    rowptr[-1] = nnz_off[-1]
+    return sys_row_ptr, rowptr, col, val
+
+
+class CgBatchPlan:
+    """Device-resident block-diagonal batch solved by one persistent launch."""
+
+    def __init__(self, sys_row_ptr, rowptr, col, val, params=None):
+        self.sys_row_ptr = np.ascontiguousarray(sys_row_ptr, dtype=np.int64)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.nsys = self.sys_row_ptr.size - 1
+        self.n = int(self.sys_row_ptr[-1])
+        self.params = params or cg_params(method=CG_PIPELINED)
+        h = _vp()
+        _check(lib.cfm_cg_batch_plan_create(self.nsys, _ptr(self.sys_row_ptr), _ptr(self.rowptr), _ptr(self.col),
+                                            _ptr(self.val), ctypes.byref(self.params), ctypes.byref(h)),
+               "cfm_cg_batch_plan_create")
+        self._h = h
+
+    def set_rhs(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib.cfm_cg_batch_set_rhs(self._h, b.ctypes.data_as(_dp)), "cfm_cg_batch_set_rhs")
+
+    def set_x0(self, x0=None):
+        if x0 is None:
+            _check(lib.cfm_cg_batch_set_x0(self._h, None), "cfm_cg_batch_set_x0")
+        else:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64)
+            _check(lib.cfm_cg_batch_set_x0(self._h, x0.ctypes.data_as(_dp)), "cfm_cg_batch_set_x0")
+
+    def solve(self, hist_cap=0):
+        """Returns (status, per-system iters, per-system status, hist[nsys, hist_cap] or None, timing)."""
+        iters = np.zeros(self.nsys, dtype=np.int32)
+        stat = np.zeros(self.nsys, dtype=np.int32)
+        hist = np.zeros((self.nsys, max(hist_cap, 1)))
+        t = Timing()
+        st = lib.cfm_cg_batch_solve(self._h, iters.ctypes.data_as(_ip), stat.ctypes.data_as(_ip),
+                                    hist.ctypes.data_as(_dp) if hist_cap > 0 else None, hist_cap, ctypes.byref(t))
+        if st not in (OK, ENOTCONV, EBREAKDOWN):
+            raise CfmError(st, "cfm_cg_batch_solve")
+        return st, iters, stat, (hist if hist_cap > 0 else None), t.as_dict()
+
+    def x(self):
+        out = np.empty(self.n)
+        _check(lib.cfm_cg_batch_get_x(self._h, out.ctypes.data_as(_dp)), "cfm_cg_batch_get_x")
+        return out
+
+    def tune(self, trial_iters=20, cache_path=None):
+        bb, bg = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.cfm_cg_batch_tune(self._h, trial_iters, cache_path.encode() if cache_path else None,
+                                     ctypes.byref(bb), ctypes.byref(bg)), "cfm_cg_batch_tune")
+        return bb.value, bg.value
+
+    def close(self):
+        if self._h:
+            lib.cfm_cg_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "cfm_common.hpp"
 
@@ -165,6 +166,88 @@ extern "C" int64_t cfm_gen_poisson3d(int nx, int ny, int nz, int *rowptr, int *c
             }
     if (rowptr) rowptr[row] = (int)k;
     return k;
+}
+
+namespace {
+// separate, non-inlined calls so that no compiler fuses sin and cos of one
+// angle into sincos(), whose last bit can differ from the separate calls
+__attribute__((noinline)) double host_cos(double a) { return std::cos(a); }
+__attribute__((noinline)) double host_sin(double a) { return std::sin(a); }
+
+struct SplitMix {
+    uint64_t s;
+    double next() {
+        s += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1.0p-53;
+    }
+};
+}  // namespace
+
+extern "C" int64_t cfm_gen_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr,
+                                     int *col, double *val, double *f) {
+    if (!dims || lmin < 1 || lmax < lmin || system_id < 0) return -1;
+    SplitMix g{seed ^ (0x9E3779B97F4A7C15ULL * ((uint64_t)system_id + 1))};
+    int L[3];
+    for (int a = 0; a < 3; ++a) {
+        const double u = g.next();
+        int v = (int)std::floor(std::exp(std::log((double)lmin) + u * (std::log((double)lmax + 1.0) - std::log((double)lmin))));
+        L[a] = v < lmin ? lmin : v > lmax ? lmax : v;
+        dims[a] = L[a];
+    }
+    const long long nx = L[0], ny = L[1], nz = L[2], n = nx * ny * nz, plane = nx * ny;
+    if (n * 7 > 0x7fffffffLL) return -1;
+    long long nnz = 0;
+    for (long long i = 0; i < n; ++i) {
+        const long long ix = i % nx, iy = (i / nx) % ny, iz = i / plane;
+        nnz += 1 + (iz > 0) + (iy > 0) + (ix > 0) + (ix + 1 < nx) + (iy + 1 < ny) + (iz + 1 < nz);
+    }
+    if (!rowptr && !col && !val && !f) return nnz;
+    std::vector<double> k((size_t)n);
+    for (long long i = 0; i < n; i += 2) {
+        const double u1 = g.next(), u2 = g.next();
+        const double r = std::sqrt(-2.0 * std::log(1.0 - u1));
+        const double th = 2.0 * 3.14159265358979323846 * u2;
+        k[i] = std::exp(r * host_cos(th));
+        if (i + 1 < n) k[i + 1] = std::exp(r * host_sin(th));
+    }
+    for (long long i = 0; i < n; ++i) {
+        const double u = g.next();
+        if (f) f[i] = 2.0 * u - 1.0;
+    }
+    long long e = 0;
+    for (long long i = 0; i < n; ++i) {
+        const long long ix = i % nx, iy = (i / nx) % ny, iz = i / plane;
+        const long long nb[6] = {iz > 0 ? i - plane : -1, iy > 0 ? i - nx : -1,         ix > 0 ? i - 1 : -1,
+                                 ix + 1 < nx ? i + 1 : -1, iy + 1 < ny ? i + nx : -1, iz + 1 < nz ? i + plane : -1};
+        double cf[6], diag = 0.0;
+        for (int d = 0; d < 6; ++d) {
+            cf[d] = nb[d] >= 0 ? (2.0 * k[i] * k[nb[d]]) / (k[i] + k[nb[d]]) : k[i];
+            diag = diag + cf[d];
+        }
+        if (rowptr) rowptr[i] = (int)e;
+        for (int d = 0; d < 7; ++d) {  // ascending columns; the diagonal sits between x-1 and x+1
+            long long c;
+            double v;
+            if (d == 3) {
+                c = i;
+                v = diag;
+            } else {
+                const int dd = d < 3 ? d : d - 1;
+                if (nb[dd] < 0) continue;
+                c = nb[dd];
+                v = -cf[dd];
+            }
+            if (col) col[e] = (int)c;
+            if (val) val[e] = v;
+            ++e;
+        }
+    }
+    if (rowptr) rowptr[n] = (int)e;
+    return e;
 }
 
 extern "C" cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count) {

@@ -34,19 +34,9 @@
 #include <cstring>
 #include <vector>
 
-#include "cfm_common.hpp"
+#include "cg_common.hpp"
 
 namespace cfm {
-
-constexpr int kChunk = 1024;
-constexpr int kThreads = 256;
-constexpr int kSlicesPerChunk = kChunk / 64;
-
-struct Sell {
-    const int *slice_ptr;  // [nslices + 1], entry offsets (multiples of 64)
-    const int *col;        // [slice_ptr[nslices]], -1 = padding
-    const void *val;       // double or float
-};
 
 struct CgArgs {
     int n;
@@ -70,57 +60,6 @@ struct CgArgs {
     GridSync *sync;
 };
 
-// ------------------------------------------------------------ SpMV row ----
-template <typename VT, typename XT>
-__device__ __forceinline__ XT sell_row(const Sell &S, const XT *__restrict__ x, int slice, int lane) {
-    const int beg = S.slice_ptr[slice];
-    const int width = (S.slice_ptr[slice + 1] - beg) >> 6;
-    const int *cp = S.col + beg + lane;
-    const VT *vp = static_cast<const VT *>(S.val) + beg + lane;
-    XT acc = XT(0);
-    for (int j = 0; j < width; j += 8) {
-        int c[8];
-        VT v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const bool ok = j + u < width;
-            c[u] = ok ? cp[(j + u) * 64] : -1;
-            v[u] = ok ? vp[(j + u) * 64] : VT(0);
-        }
-        XT xv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : XT(0);
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (c[u] >= 0) acc = fma((XT)v[u], xv[u], acc);
-    }
-    return acc;
-}
-
-// A workgroup is SUB sub-blocks of 256 threads; every sub-block forms its
-// sums with the same fixed tree: xor butterfly per wavefront, then
-// (w0 + w1) + (w2 + w3) over its four wavefronts.  lds holds 4*SUB doubles.
-template <int SUB>
-__device__ __forceinline__ double sub_sum(double v, double *lds) {
-    v = wave_butterfly(v);
-    __syncthreads();  // lds may still be read from a previous call
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-    __syncthreads();
-    const int w0 = (threadIdx.x >> 8) * 4;
-    return (lds[w0] + lds[w0 + 1]) + (lds[w0 + 2] + lds[w0 + 3]);
-}
-
-// Reduce nchunks partials (fixed order) -> same value in every thread: each
-// sub-block's thread t adds partials t, t+256, ... and the sub-block tree
-// combines them (every sub-block computes the identical value).
-template <int SUB>
-__device__ __forceinline__ double reduce_partials(const double *part, int nchunks, double *lds) {
-    double a = 0.0;
-    for (int j = threadIdx.x & (kThreads - 1); j < nchunks; j += kThreads) a = a + part[j];
-    return sub_sum<SUB>(a, lds);
-}
-
-__device__ __forceinline__ bool is_finite(double v) { return v - v == 0.0; }
 
 // Iterate the rows this thread owns: sub-block `sub_` of workgroup b takes
 // chunk c = b*SUB + sub_ (+ multiples of gridDim.x*SUB); inside a chunk its

@@ -175,12 +175,53 @@ cfm_status cfm_cg_solve(int n, int nnz, const int *rowptr, const int *col, const
                         int index_base, const double *b, double *x_io, const cfm_cg_params *p,
                         double *hist, int hist_cap, int *iters, cfm_timing *t);
 
+/* ------------------------------------------------------------ batched --- */
+/* Many independent SPD systems in ONE persistent launch (Pipelined CG, one
+ * device-wide barrier per iteration, per-system scalars, tolerance and
+ * status).  No counterpart in the reference, which solves one matrix per run.
+ * The systems form one block-diagonal CSR: rows sys_row_ptr[s] ..
+ * sys_row_ptr[s+1]-1 belong to system s and their columns must stay inside
+ * that range (0-based).  params: method must be CFM_CG_PIPELINED and mode
+ * CFM_CG_PERSISTENT; tol / max_iter / restart_quirk apply to every system.
+ * A batch member gives bit for bit the result of solving it alone with
+ * cfm_cg_plan_* (same method and flags). */
+typedef struct cfm_cg_batch_plan cfm_cg_batch_plan;
+cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_ptr, const int *rowptr,
+                                    const int *col, const double *val, const cfm_cg_params *p,
+                                    cfm_cg_batch_plan **out);
+cfm_status cfm_cg_batch_set_rhs(cfm_cg_batch_plan *plan, const double *b);  /* packed, all systems */
+cfm_status cfm_cg_batch_set_x0(cfm_cg_batch_plan *plan, const double *x0);  /* NULL = zero */
+/* iters/status (may be NULL) receive one entry per system; hist (may be
+ * NULL) receives <r_k,r_k> as hist[s * hist_cap + k].  Returns CFM_OK when
+ * every system converged, else the worst per-system status. */
+cfm_status cfm_cg_batch_solve(cfm_cg_batch_plan *plan, int *iters, int *status, double *hist,
+                              int hist_cap, cfm_timing *t);
+cfm_status cfm_cg_batch_get_x(cfm_cg_batch_plan *plan, double *x);
+/* Autotuner: times trial_iters fixed iterations for each workgroup shape
+ * (256/512/1024 threads) and grid size (1..occupancy workgroups per CU) and
+ * keeps the fastest; cache_path (may be NULL) stores/looks up the winner per
+ * (device, system count, row count). */
+cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *plan, int trial_iters, const char *cache_path,
+                             int *best_block, int *best_grid);
+void cfm_cg_batch_plan_destroy(cfm_cg_batch_plan *plan);
+
 /* ---------------------------------------------------------- generators --- */
 /* 3D 7-point Poisson on an nx*ny*nz grid, Dirichlet, rows ix + nx*(iy + ny*iz),
  * columns ascending, diagonal 6, off-diagonals -1 (the CG workload; the
  * reference generates a random tridiagonal instead, CG/src/common.cu:46-76).
  * Returns nnz (>= 0) or -1; pass NULL arrays to query the size. */
 int64_t cfm_gen_poisson3d(int nx, int ny, int nz, int *rowptr, int *col, double *val);
+/* Variable-coefficient system -div(k grad u) = f, u = 0 outside the box:
+ * stream (seed, system_id) draws Lx, Ly, Lz = floor(exp(U(ln lmin, ln(lmax+1))))
+ * clipped to [lmin, lmax], then k per cell = exp(N(0,1)) (Box-Muller on
+ * uniforms: r = sqrt(-2 ln(1-u1)), z = r cos(2 pi u2), r sin(2 pi u2)), then
+ * f per cell = 2u - 1.  7-point rows (ascending columns z-1, y-1, x-1, diag,
+ * x+1, y+1, z+1): off-diagonal -2 k_i k_j / (k_i + k_j); a face on the box
+ * boundary contributes k_i; the diagonal sums the six face coefficients in
+ * that order.  dims receives (Lx, Ly, Lz).  Returns nnz; NULL arrays query
+ * the sizes (dims is always written). */
+int64_t cfm_gen_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr,
+                          int *col, double *val, double *f);
 /* SplitMix64 stream (seed, stream_id) -> count doubles in [0,1). */
 cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count);
 

@@ -346,3 +346,82 @@ int64_t orc_poisson3d(int nx, int ny, int nz, int *rowptr, int *col, double *val
     if (rowptr) rowptr[row] = (int)k;
     return k;
 }
+
+/* Variable-coefficient -div(k grad u) = f generator (the batched and learned
+ * warm-start workloads).  The definition is the build's own (no reference
+ * counterpart): dims ~ floor(exp(U(ln lmin, ln(lmax+1)))) clipped; k =
+ * exp(N(0,1)) by Box-Muller; f = 2u - 1; harmonic-mean faces, boundary faces
+ * k_i; diagonal = sum of the six face coefficients in z-,y-,x-,x+,y+,z+ order. */
+/* separate, non-inlined calls: a compiler may otherwise fuse sin and cos of
+ * the same angle into sincos(), whose last bit can differ */
+__attribute__((noinline)) static double orc_cos(double a) { return cos(a); }
+__attribute__((noinline)) static double orc_sin(double a) { return sin(a); }
+
+int64_t orc_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr, int *col,
+                      double *val, double *f) {
+    if (!dims || lmin < 1 || lmax < lmin || system_id < 0) return -1;
+    uint64_t st = orc_stream_state(seed, (uint64_t)system_id);
+    for (int a = 0; a < 3; ++a) {
+        const double u = (double)(orc_splitmix64_next(&st) >> 11) * 0x1.0p-53;
+        const double lo = log((double)lmin), hi = log((double)lmax + 1.0);
+        int v = (int)floor(exp(lo + u * (hi - lo)));
+        if (v < lmin) v = lmin;
+        if (v > lmax) v = lmax;
+        dims[a] = v;
+    }
+    const long nx = dims[0], ny = dims[1], nz = dims[2], n = nx * ny * nz, plane = nx * ny;
+    int64_t nnz = 0;
+    for (long iz = 0; iz < nz; ++iz)
+        for (long iy = 0; iy < ny; ++iy)
+            for (long ix = 0; ix < nx; ++ix)
+                nnz += 1 + (iz > 0) + (iy > 0) + (ix > 0) + (ix < nx - 1) + (iy < ny - 1) + (iz < nz - 1);
+    if (!rowptr && !col && !val && !f) return nnz;
+    double *k = (double *)malloc(sizeof(double) * n);
+    if (!k) return -1;
+    const double pi = 3.14159265358979323846;
+    for (long i = 0; i < n; i += 2) {
+        const double u1 = (double)(orc_splitmix64_next(&st) >> 11) * 0x1.0p-53;
+        const double u2 = (double)(orc_splitmix64_next(&st) >> 11) * 0x1.0p-53;
+        const double r = sqrt(-2.0 * log(1.0 - u1));
+        k[i] = exp(r * orc_cos(2.0 * pi * u2));
+        if (i + 1 < n) k[i + 1] = exp(r * orc_sin(2.0 * pi * u2));
+    }
+    for (long i = 0; i < n; ++i) {
+        const double u = (double)(orc_splitmix64_next(&st) >> 11) * 0x1.0p-53;
+        if (f) f[i] = 2.0 * u - 1.0;
+    }
+    int64_t e = 0;
+    long row = 0;
+    for (long iz = 0; iz < nz; ++iz)
+        for (long iy = 0; iy < ny; ++iy)
+            for (long ix = 0; ix < nx; ++ix, ++row) {
+                /* neighbours in the fixed face order z-, y-, x-, x+, y+, z+ */
+                const long nb[6] = {iz > 0 ? row - plane : -1, iy > 0 ? row - nx : -1, ix > 0 ? row - 1 : -1,
+                                    ix < nx - 1 ? row + 1 : -1, iy < ny - 1 ? row + nx : -1,
+                                    iz < nz - 1 ? row + plane : -1};
+                double c[6], diag = 0.0;
+                for (int d = 0; d < 6; ++d) {
+                    c[d] = nb[d] >= 0 ? (2.0 * k[row] * k[nb[d]]) / (k[row] + k[nb[d]]) : k[row];
+                    diag = diag + c[d];
+                }
+                if (rowptr) rowptr[row] = (int)e;
+                for (int d = 0; d < 3; ++d)
+                    if (nb[d] >= 0) {
+                        if (col) col[e] = (int)nb[d];
+                        if (val) val[e] = -c[d];
+                        ++e;
+                    }
+                if (col) col[e] = (int)row;
+                if (val) val[e] = diag;
+                ++e;
+                for (int d = 3; d < 6; ++d)
+                    if (nb[d] >= 0) {
+                        if (col) col[e] = (int)nb[d];
+                        if (val) val[e] = -c[d];
+                        ++e;
+                    }
+            }
+    if (rowptr) rowptr[row] = (int)e;
+    free(k);
+    return e;
+}

@@ -136,6 +136,11 @@ void orc_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count);
  * Returns nnz; rowptr/col/val may be NULL to query the size. */
 int64_t orc_poisson3d(int nx, int ny, int nz, int *rowptr, int *col, double *val);
 
+/* Variable-coefficient -div(k grad u) = f system of the batched / warm-start
+ * workloads (definition in cg_oracle.c).  Returns nnz; dims receives L. */
+int64_t orc_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr, int *col,
+                      double *val, double *f);
+
 #ifdef __cplusplus
 }
 #endif

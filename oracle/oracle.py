@@ -70,6 +70,8 @@ def lib():
         L.orc_uniform.argtypes = [c_u64, c_u64, _dp, c_i64]
         L.orc_poisson3d.argtypes = [c_int, c_int, c_int, _ip, _ip, _dp]
         L.orc_poisson3d.restype = c_i64
+        L.orc_varcoef3d.argtypes = [c_u64, c_int, c_int, c_int, _ip, _ip, _ip, _dp, _dp]
+        L.orc_varcoef3d.restype = c_i64
         _lib = L
     return _lib
 
@@ -210,3 +212,18 @@ def poisson3d(nx, ny, nz):
     val = np.empty(nnz, dtype=np.float64)
     L.orc_poisson3d(nx, ny, nz, _ptr(rowptr, _ip), _ptr(col, _ip), _ptr(val, _dp))
     return Csr(rowptr, col, val)
+
+
+def varcoef3d(seed, system_id, lmin, lmax):
+    """Returns (Csr, f, dims)."""
+    L = lib()
+    dims = np.zeros(3, dtype=np.int32)
+    nnz = L.orc_varcoef3d(seed, system_id, lmin, lmax, _ptr(dims, _ip), None, None, None, None)
+    n = int(dims.prod())
+    rowptr = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    f = np.empty(n, dtype=np.float64)
+    L.orc_varcoef3d(seed, system_id, lmin, lmax, _ptr(dims, _ip), _ptr(rowptr, _ip), _ptr(col, _ip),
+                    _ptr(val, _dp), _ptr(f, _dp))
+    return Csr(rowptr, col, val), f, tuple(int(d) for d in dims)

@@ -82,3 +82,21 @@ def test_no_device_is_an_error_not_a_fallback(cfm):
     assert cfm.lib.cfm_jacobi_plan_create(ctypes.byref(p), ctypes.byref(h)) == cfm.EDEVICE
     with pytest.raises(cfm.CfmError):
         cfm.jacobi(np.zeros((8, 8, 8)), 1)
+
+
+def test_varcoef_generator_matches_oracle(cfm, orc):
+    for sid in range(12):
+        A, f, dims = orc.varcoef3d(42, sid, 4, 11)
+        rp, col, val, f2, d2 = cfm.varcoef3d(42, sid, 4, 11)
+        assert d2 == dims
+        assert np.array_equal(rp, A.rowptr) and np.array_equal(col, A.col)
+        assert np.array_equal(val, A.val) and np.array_equal(f2, f)
+
+
+def test_block_diag_layout(cfm):
+    s1 = (np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32), np.array([2.0, 3.0]))
+    s2 = (np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([4.0, -1, -1, 4.0]))
+    srp, rp, col, val = cfm.block_diag([s1, s2])
+    assert srp.tolist() == [0, 2, 4]
+    assert rp.tolist() == [0, 1, 2, 4, 6]
+    assert col.tolist() == [0, 1, 2, 3, 2, 3]

@@ -1,0 +1,165 @@
+"""Batched CG on the GPU (one persistent launch for many systems).
+
+Every batch member must equal, bit for bit, the oracle's Pipelined CG on that
+system alone (same chunk-tree dot products, same fused updates), whatever the
+workgroup shape, grid size or the other members of the batch; per-system
+tolerance, iteration limits and breakdown must not leak between systems.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(gpu, seed, nsys, lmin, lmax):
+    out = []
+    for sid in range(nsys):
+        rp, col, val, f, dims = gpu.varcoef3d(seed, sid, lmin, lmax)
+        out.append((rp, col, val, f))
+    return out
+
+
+def _batch(gpu, systems, **kw):
+    srp, rp, col, val = gpu.block_diag([s[:3] for s in systems])
+    b = np.concatenate([s[3] for s in systems])
+    p = gpu.cg_params(method=gpu.CG_PIPELINED, **kw)
+    return srp, rp, col, val, b, p
+
+
+def _check_members(orc, systems, srp, x, iters, hist, tol, max_iter, quirk=True, x0=None):
+    for i, (rp, col, val, f) in enumerate(systems):
+        r0, r1 = srp[i], srp[i + 1]
+        st, ox, oh, oit = orc.cg(orc.Csr(rp, col, val), f, x0=None if x0 is None else x0[r0:r1], tol=tol,
+                                 max_iter=max_iter, variant="pipelined", quirk=quirk)
+        assert iters[i] == oit, (i, iters[i], oit)
+        assert np.array_equal(x[r0:r1], ox), i
+        assert np.array_equal(hist[i, : oit + 1], oh), i
+
+
+@pytest.mark.parametrize("quirk", [1, 0])
+def test_batch_members_equal_oracle(gpu, orc, quirk):
+    systems = _systems(gpu, 3, 12, 5, 14)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-10, max_iter=2000, restart_quirk=quirk)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=2001)
+        x = plan.x()
+    assert st == gpu.OK and np.all(stat == gpu.OK)
+    _check_members(orc, systems, srp, x, iters, hist, 1e-10, 2000, bool(quirk))
+
+
+@pytest.mark.parametrize("block,grid", [(256, 0), (512, 0), (1024, 0), (256, 1), (1024, 3), (512, 7)])
+def test_batch_shapes_do_not_change_bits(gpu, orc, block, grid):
+    systems = _systems(gpu, 5, 9, 9, 20)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-9, max_iter=3000, block_threads=block,
+                                     grid_blocks=grid)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=3001)
+        x = plan.x()
+    assert st == gpu.OK
+    _check_members(orc, systems, srp, x, iters, hist, 1e-9, 3000)
+
+
+def test_batch_equals_solo_gpu(gpu):
+    systems = _systems(gpu, 8, 6, 10, 24)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-8, max_iter=5000)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=0)
+        x = plan.x()
+    for i, (srp_i, s) in enumerate(zip(srp, systems)):
+        sst, sx, sh, sit, st_ = gpu.cg(*s[:3], s[3], method=gpu.CG_PIPELINED, tol=1e-8, max_iter=5000)
+        assert sit == iters[i] and np.array_equal(sx, x[srp[i]:srp[i + 1]])
+
+
+def test_batch_fixed_iterations_and_limits(gpu, orc):
+    systems = _systems(gpu, 9, 5, 6, 12)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=0.0, max_iter=25)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=26)
+        x = plan.x()
+    assert st == gpu.OK and np.all(iters == 25)
+    _check_members(orc, systems, srp, x, iters, hist, 0.0, 25)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-12, max_iter=7)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve()
+    assert st == gpu.ENOTCONV and np.all(stat == gpu.ENOTCONV) and np.all(iters == 7)
+
+
+def test_batch_breakdown_is_contained(gpu, orc):
+    systems = _systems(gpu, 4, 4, 6, 10)
+    n = 700  # an indefinite diagonal system in the middle of the batch
+    bad = (np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32),
+           np.where(np.arange(n) % 2 == 0, 1.0, -1.0), np.ones(n))
+    allsys = systems[:2] + [bad] + systems[2:]
+    srp, rp, col, val, b, p = _batch(gpu, allsys, tol=1e-10, max_iter=2000)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=2001)
+        x = plan.x()
+    assert st == gpu.EBREAKDOWN
+    assert stat[2] == gpu.EBREAKDOWN and np.all(np.delete(stat, 2) == gpu.OK)
+    good = [0, 1, 3, 4]
+    for i in good:
+        rp_, col_, val_, f_ = allsys[i]
+        st_o, ox, oh, oit = orc.cg(orc.Csr(rp_, col_, val_), f_, tol=1e-10, max_iter=2000, variant="pipelined")
+        assert iters[i] == oit and np.array_equal(x[srp[i]:srp[i + 1]], ox)
+
+
+def test_batch_nonzero_x0(gpu, orc):
+    systems = _systems(gpu, 12, 5, 6, 12)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-10, max_iter=2000)
+    x0 = gpu.uniform(1, 2, int(srp[-1]))
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(x0)
+        st, iters, stat, hist, t = plan.solve(hist_cap=2001)
+        x = plan.x()
+    _check_members(orc, systems, srp, x, iters, hist, 1e-10, 2000, x0=x0)
+
+
+def test_batch_autotune_and_cache(gpu, orc, tmp_path):
+    systems = _systems(gpu, 6, 16, 8, 24)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-9, max_iter=3000)
+    cache = str(tmp_path / "tune.txt")
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        blk, grid = plan.tune(trial_iters=10, cache_path=cache)
+        assert blk in (256, 512, 1024) and grid >= 1
+        st, iters, stat, hist, t = plan.solve(hist_cap=3001)
+        x = plan.x()
+        assert t["block_threads"] == blk and t["grid_blocks"] == grid
+    _check_members(orc, systems, srp, x, iters, hist, 1e-9, 3000)
+    with open(cache) as fh:
+        assert len(fh.read().strip().splitlines()) == 1
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:  # cache hit, no new line
+        assert plan.tune(trial_iters=10, cache_path=cache) == (blk, grid)
+    with open(cache) as fh:
+        assert len(fh.read().strip().splitlines()) == 1
+
+
+def test_batch_converges_true_residual(gpu):
+    systems = _systems(gpu, 0, 24, 16, 40)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-8, max_iter=10000)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve()
+        x = plan.x()
+    assert st == gpu.OK
+    for i, (rp_, col_, val_, f_) in enumerate(systems):
+        M = sp.csr_matrix((val_, col_, rp_))
+        xi = x[srp[i]:srp[i + 1]]
+        r = f_ - M @ xi
+        assert np.linalg.norm(r) <= 2e-8 * np.linalg.norm(f_)

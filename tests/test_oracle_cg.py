@@ -178,3 +178,51 @@ def test_ir_reaches_fp64_tolerance(orc):
     assert r @ r <= 1e-20 * (b @ b)
     # each refinement gains about the inner tolerance
     assert all(hist[k + 1] < 1e-5 * hist[k] for k in range(outer - 1))
+
+
+def test_varcoef_generator_properties(orc):
+    # the definition is the build's own; pin its mathematics: dims in range,
+    # exact symmetry (harmonic means are symmetric bit for bit), positive
+    # definiteness, zero row sums away from the box faces, k = exp(N(0,1)).
+    logs = []
+    for sid in range(6):
+        A, f, dims = orc.varcoef3d(11, sid, 5, 9)
+        assert all(5 <= d <= 9 for d in dims)
+        n = A.n
+        assert n == dims[0] * dims[1] * dims[2] and f.size == n
+        M = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(n, n))
+        assert (M != M.T).nnz == 0
+        dense = M.toarray()
+        assert np.linalg.eigvalsh(dense).min() > 0
+        nx, ny, nz = dims
+        iz, iy, ix = np.unravel_index(np.arange(n), (nz, ny, nx))
+        inner = (ix > 0) & (ix < nx - 1) & (iy > 0) & (iy < ny - 1) & (iz > 0) & (iz < nz - 1)
+        rows = np.asarray(M.sum(axis=1)).ravel()
+        assert np.allclose(rows[inner], 0.0, atol=1e-12 * np.abs(M.diagonal()).max())
+        assert np.all(rows[~inner] > 0)
+        assert f.min() >= -1.0 and f.max() < 1.0
+    # a 1x1x1 box: every face is a boundary face, diagonal = 6 k
+    A, f, dims = orc.varcoef3d(3, 0, 1, 1)
+    assert dims == (1, 1, 1) and A.nnz == 1 and A.val[0] > 0
+    # log k ~ N(0,1): recover k from a corner cell's diagonal on large boxes
+    ks = []
+    for sid in range(30):
+        A, f, dims = orc.varcoef3d(5, sid, 1, 1)
+        ks.append(np.log(A.val[0] / 6.0))
+    ks = np.array(ks)
+    assert abs(ks.mean()) < 0.6 and 0.5 < ks.std() < 1.6
+
+
+def test_varcoef_dims_are_log_uniform(orc):
+    # dims only (no matrix): the size query draws only the three lengths
+    import ctypes
+    L = orc.lib()
+    d = np.zeros(3, dtype=np.int32)
+    draws = []
+    for sid in range(600):
+        L.orc_varcoef3d(0, sid, 32, 96, d.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None, None, None)
+        draws += d.tolist()
+    draws = np.array(draws)
+    assert draws.min() >= 32 and draws.max() <= 96
+    # log-uniform: the median is near sqrt(32*97) = 55.7
+    assert 50 <= np.median(draws) <= 62

@@ -37,6 +37,8 @@ EXPORTS = (
     "cfm_cg_plan_get_x", "cfm_cg_plan_destroy", "cfm_cg_solve", "cfm_gen_poisson3d", "cfm_gen_uniform",
     "cfm_gen_varcoef3d", "cfm_cg_batch_plan_create", "cfm_cg_batch_set_rhs", "cfm_cg_batch_set_x0",
     "cfm_cg_batch_solve", "cfm_cg_batch_get_x", "cfm_cg_batch_tune", "cfm_cg_batch_plan_destroy",
+    "cfm_unet_param_count", "cfm_unet_create", "cfm_unet_destroy", "cfm_unet_init_params", "cfm_unet_set_params",
+    "cfm_unet_get_params", "cfm_unet_forward", "cfm_unet_gradient", "cfm_unet_train_step",
 )
 
 
@@ -119,6 +121,18 @@ def _load():
     L.cfm_cg_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, _ip, _ip]
     L.cfm_cg_batch_plan_destroy.argtypes = [_vp]
     L.cfm_cg_batch_plan_destroy.restype = None
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.cfm_unet_param_count.argtypes = []
+    L.cfm_unet_param_count.restype = ctypes.c_size_t
+    L.cfm_unet_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.cfm_unet_destroy.argtypes = [_vp]
+    L.cfm_unet_destroy.restype = None
+    L.cfm_unet_init_params.argtypes = [ctypes.c_uint64, _fp]
+    L.cfm_unet_set_params.argtypes = [_vp, _fp]
+    L.cfm_unet_get_params.argtypes = [_vp, _fp]
+    L.cfm_unet_forward.argtypes = [_vp, _fp, _fp]
+    L.cfm_unet_gradient.argtypes = [_vp, _fp, _fp, _fp, _dp]
+    L.cfm_unet_train_step.argtypes = [_vp, _fp, _fp, ctypes.c_float, _dp]
     return L
 
 
@@ -443,6 +457,106 @@ class CgBatchPlan:
     def close(self):
         if self._h:
             lib.cfm_cg_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------- learned warm start --
+
+# (Cin, Cout) of the ten 3x3x3 convolutions, in parameter order.
+UNET_CONVS = ((2, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (32, 32), (48, 16), (16, 16))
+UNET_HEAD_IN = 16
+
+
+def unet_param_count():
+    return int(lib.cfm_unet_param_count())
+
+
+def unet_init_params(seed):
+    out = np.empty(unet_param_count(), dtype=np.float32)
+    _check(lib.cfm_unet_init_params(seed, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+           "cfm_unet_init_params")
+    return out
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class UNet:
+    """The warm-start network on the device: forward, gradient and Adam steps
+    on batches of n^3 grids (input [batch, 2, n, n, n], output [batch, n, n, n])."""
+
+    def __init__(self, n, batch, params=None, seed=0):
+        self.n, self.batch = int(n), int(batch)
+        h = _vp()
+        _check(lib.cfm_unet_create(self.n, self.batch, ctypes.byref(h)), "cfm_unet_create")
+        self._h = h
+        self.set_params(unet_init_params(seed) if params is None else params)
+
+    def _in(self, x):
+        x = _f32(x)
+        if x.size != self.batch * 2 * self.n ** 3:
+            raise ValueError("input must hold batch * 2 * n^3 values")
+        return x
+
+    def _tgt(self, y):
+        y = _f32(y)
+        if y.size != self.batch * self.n ** 3:
+            raise ValueError("target must hold batch * n^3 values")
+        return y
+
+    def set_params(self, params):
+        p = _f32(params)
+        if p.size != unet_param_count():
+            raise ValueError("parameter vector has the wrong length")
+        _check(lib.cfm_unet_set_params(self._h, _fptr(p)), "cfm_unet_set_params")
+
+    def params(self):
+        out = np.empty(unet_param_count(), dtype=np.float32)
+        _check(lib.cfm_unet_get_params(self._h, _fptr(out)), "cfm_unet_get_params")
+        return out
+
+    def forward(self, x):
+        x = self._in(x)
+        out = np.empty((self.batch, self.n, self.n, self.n), dtype=np.float32)
+        _check(lib.cfm_unet_forward(self._h, _fptr(x), _fptr(out)), "cfm_unet_forward")
+        return out
+
+    def gradient(self, x, target):
+        """Returns (loss, flat gradient)."""
+        x, y = self._in(x), self._tgt(target)
+        g = np.empty(unet_param_count(), dtype=np.float32)
+        loss = ctypes.c_double(0)
+        _check(lib.cfm_unet_gradient(self._h, _fptr(x), _fptr(y), _fptr(g), ctypes.byref(loss)),
+               "cfm_unet_gradient")
+        return loss.value, g
+
+    def train_step(self, x, target, lr):
+        x, y = self._in(x), self._tgt(target)
+        loss = ctypes.c_double(0)
+        _check(lib.cfm_unet_train_step(self._h, _fptr(x), _fptr(y), float(lr), ctypes.byref(loss)),
+               "cfm_unet_train_step")
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib.cfm_unet_destroy(self._h)
             self._h = None
 
     def __enter__(self):

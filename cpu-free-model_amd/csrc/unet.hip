@@ -1,0 +1,780 @@
+// Learned warm start: a 3-level 3D U-Net (16/32/64 channels, 3x3x3 convs)
+// with hand-written forward and backward kernels on BF16 MFMA (FP32
+// accumulate), trained on the GPU with Adam.  No counterpart in the
+// reference: it produces initial guesses x0 for the CG solvers.
+//
+// Tensors are channels-last (NDHWC) BF16; spatial index p = ((b*n+z)*n+y)*n+x.
+// A 3x3x3 convolution is an implicit GEMM: Y[p][co] = sum_k W[co][k] Xs[p][k]
+// with k = tap*Cin + ci and Xs[p][k] = X[p + offset(tap)][ci] (zero outside).
+// One v_mfma_f32_16x16x32_bf16 handles 16 output channels x 16 points x 32
+// k; lane l holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and
+// D[4(l/16)+i][l%16] (layout checked on the device).  Channel counts are
+// multiples of 8, so a lane's 8 k-values are 8 consecutive channels of one
+// tap: one 16-byte load.
+//
+// Backward data is the same kernel with the weights flipped in space and
+// transposed in channels; backward weights is a split-K MFMA GEMM over point
+// slabs followed by a fixed-order reduction of the slab partials.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cfm_common.hpp"
+
+namespace cfm {
+namespace unet {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 v) { return (float)v; }
+__device__ __forceinline__ bf16 f2bf(float v) { return (bf16)v; }  // round to nearest even
+
+__device__ __forceinline__ int tap_off(int tap, int &dz, int &dy, int &dx) {
+    dz = tap / 9 - 1;
+    dy = (tap / 3) % 3 - 1;
+    dx = tap % 3 - 1;
+    return 0;
+}
+
+// ------------------------------------------------------------- conv fwd ---
+// Y = conv3x3x3(X) + bias, optional ReLU.  W is [Cout][Kp] BF16 (k = tap*Cin+ci,
+// zero padded to Kp, a multiple of 32).  MT = Cout / 16 tiles per wavefront.
+template <int MT>
+__global__ void __launch_bounds__(256) conv3_fwd_kernel(const bf16 *__restrict__ X, int Cin,
+                                                        const bf16 *__restrict__ W, int Kp,
+                                                        const float *__restrict__ bias, bf16 *__restrict__ Y,
+                                                        float *__restrict__ Yf, int n, int npts, int relu) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int p0 = tile * 16;
+    if (p0 >= npts) return;  // uniform per wavefront
+    const int nl = lane & 15, g = lane >> 4;
+    const int p = p0 + nl;
+    const bool pvalid = p < npts;  // npts need not be a multiple of 16
+    const int x = p % n, y = (p / n) % n, z = (p / (n * n)) % n, b = p / (n * n * n);
+    const int K = 27 * Cin;
+    f32x4 acc[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < Kp; kb += 32) {
+        const int k = kb + 8 * g;
+        bf16x8 bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = (bf16)0.0f;
+        if (pvalid && k < K) {
+            const int tap = k / Cin, ci = k - tap * Cin;
+            int dz, dy, dx;
+            tap_off(tap, dz, dy, dx);
+            const int zz = z + dz, yy = y + dy, xx = x + dx;
+            if (zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n) {
+                const size_t q = (((size_t)b * n + zz) * n + yy) * n + xx;
+                bv = *reinterpret_cast<const bf16x8 *>(X + q * Cin + ci);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8 *>(W + (size_t)(m * 16 + nl) * Kp + k);
+            acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[m], 0, 0, 0);
+        }
+    }
+    // D[4g+i][nl]: output channel m*16 + 4g + i of point p0 + nl
+    const int Cout = MT * 16;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = m * 16 + 4 * g + i;
+            if (!pvalid) continue;
+            float v = acc[m][i] + (bias ? bias[co] : 0.f);
+            if (relu && v < 0.f) v = 0.f;
+            if (Y) Y[(size_t)p * Cout + co] = f2bf(v);
+            if (Yf) Yf[(size_t)p * Cout + co] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------ conv weight grad ---
+// partial[slab][co][k] = sum over points of slab: dZ[p][co] * Xs[p][k]
+// (M = 16 output channels, N = 16 k, K = 32 points per MFMA).  Each
+// wavefront owns one (co tile, k tile) pair over one slab of `slab` points.
+__global__ void __launch_bounds__(256) conv3_wgrad_kernel(const bf16 *__restrict__ X, int Cin,
+                                                          const bf16 *__restrict__ dZ, int Cout, int Kp,
+                                                          float *__restrict__ partial, int n, int npts, int slab,
+                                                          int ntile_k) {
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ntile = (Cout / 16) * ntile_k;
+    const int s = blockIdx.y;
+    if (wid >= ntile) return;
+    const int mt = wid / ntile_k, kt = wid - mt * ntile_k;
+    const int nl = lane & 15, g = lane >> 4;
+    const int co = mt * 16 + nl;         // A row (lane's m)
+    const int kcol = kt * 16 + nl;       // B column (lane's n)
+    const int K = 27 * Cin;
+    int tap = 0, ci = 0, dz = 0, dy = 0, dx = 0;
+    const bool kvalid = kcol < K;
+    if (kvalid) {
+        tap = kcol / Cin;
+        ci = kcol - tap * Cin;
+        tap_off(tap, dz, dy, dx);
+    }
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int pbeg = s * slab, pend = min(npts, pbeg + slab);
+    for (int pb = pbeg; pb < pend; pb += 32) {
+        bf16x8 av, bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int p = pb + 8 * g + j;
+            float a = 0.f, bb = 0.f;
+            if (p < pend) {
+                a = bf2f(dZ[(size_t)p * Cout + co]);
+                if (kvalid) {
+                    const int x = p % n, y = (p / n) % n, z = (p / (n * n)) % n, b = p / (n * n * n);
+                    const int zz = z + dz, yy = y + dy, xx = x + dx;
+                    if (zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n)
+                        bb = bf2f(X[((((size_t)b * n + zz) * n + yy) * n + xx) * Cin + ci]);
+                }
+            }
+            av[j] = f2bf(a);
+            bv[j] = f2bf(bb);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    // D[4g+i][nl] = dW[mt*16 + 4g + i][kt*16 + nl]
+    float *out = partial + (size_t)s * Cout * Kp;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[(size_t)(mt * 16 + 4 * g + i) * Kp + kt * 16 + nl] = acc[i];
+}
+
+// out[i] = sum_s partial[s][i] (fixed order), i < count
+__global__ void reduce_slabs_kernel(const float *__restrict__ partial, int nslab, size_t count,
+                                    float *__restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float a = 0.f;
+    for (int s = 0; s < nslab; ++s) a += partial[(size_t)s * count + i];
+    out[i] = a;
+}
+
+// per-slab channel sums of dZ (bias gradient), partial[slab][c]
+__global__ void chan_sum_kernel(const bf16 *__restrict__ dZ, int C, int npts, int slab, float *__restrict__ partial) {
+    const int s = blockIdx.x;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        float a = 0.f;
+        const int pend = min(npts, (s + 1) * slab);
+        for (int p = s * slab; p < pend; ++p) a += bf2f(dZ[(size_t)p * C + c]);
+        partial[(size_t)s * C + c] = a;
+    }
+}
+
+// --------------------------------------------------------- elementwise ----
+__global__ void relu_bwd_kernel(const bf16 *__restrict__ Y, const bf16 *__restrict__ dY, bf16 *__restrict__ dZ,
+                                size_t count) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < count) dZ[i] = bf2f(Y[i]) > 0.f ? dY[i] : (bf16)0.0f;
+}
+
+// 2x2x2 average pooling: n -> n/2
+__global__ void pool_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y, int C, int n, int nb) {
+    const int m = n / 2;
+    const size_t total = (size_t)nb * m * m * m * C;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = i % C;
+    size_t q = i / C;
+    const int x = q % m, y = (q / m) % m, z = (q / ((size_t)m * m)) % m, b = q / ((size_t)m * m * m);
+    float a = 0.f;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                a += bf2f(X[((((size_t)b * n + 2 * z + dz) * n + 2 * y + dy) * n + 2 * x + dx) * C + c]);
+    Y[i] = f2bf(a * 0.125f);
+}
+
+__global__ void pool_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int n, int nb) {
+    const size_t total = (size_t)nb * n * n * n * C;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = i % C;
+    size_t q = i / C;
+    const int m = n / 2;
+    const int x = q % n, y = (q / n) % n, z = (q / ((size_t)n * n)) % n, b = q / ((size_t)n * n * n);
+    dX[i] = f2bf(bf2f(dY[((((size_t)b * m + z / 2) * m + y / 2) * m + x / 2) * C + c]) * 0.125f);
+}
+
+// nearest upsampling m -> 2m, written into channels [off, off+C) of a buffer with Ct channels
+__global__ void up_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y, int C, int Ct, int off, int m,
+                              int nb) {
+    const int n = 2 * m;
+    const size_t total = (size_t)nb * n * n * n * C;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = i % C;
+    size_t q = i / C;
+    const int x = q % n, y = (q / n) % n, z = (q / ((size_t)n * n)) % n, b = q / ((size_t)n * n * n);
+    Y[q * Ct + off + c] = X[((((size_t)b * m + z / 2) * m + y / 2) * m + x / 2) * C + c];
+}
+
+// gradient of nearest upsampling: sum of the 8 children (from channels [off, off+C) of dY)
+__global__ void up_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int Ct, int off, int m,
+                              int nb) {
+    const int n = 2 * m;
+    const size_t total = (size_t)nb * m * m * m * C;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int c = i % C;
+    size_t q = i / C;
+    const int x = q % m, y = (q / m) % m, z = (q / ((size_t)m * m)) % m, b = q / ((size_t)m * m * m);
+    float a = 0.f;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                a += bf2f(dY[((((size_t)b * n + 2 * z + dz) * n + 2 * y + dy) * n + 2 * x + dx) * Ct + off + c]);
+    dX[i] = f2bf(a);
+}
+
+// copy C channels between buffers with different channel counts / offsets
+__global__ void chan_copy_kernel(const bf16 *__restrict__ src, int Cs, int offs, bf16 *__restrict__ dst, int Cd,
+                                 int offd, int C, size_t npts) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= npts * C) return;
+    const size_t p = i / C;
+    const int c = i % C;
+    dst[p * Cd + offd + c] = src[p * Cs + offs + c];
+}
+
+// add two gradient contributions: a += b (elementwise)
+__global__ void add_kernel(bf16 *__restrict__ a, const bf16 *__restrict__ b, size_t count) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < count) a[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+
+// input packing: host-order float planes [b][2][n^3] -> NDHWC BF16 with 8 channels (2 used)
+__global__ void pack_input_kernel(const float *__restrict__ in, bf16 *__restrict__ X, int n3, int nb) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)nb * n3) return;
+    const size_t b = i / n3, q = i % n3;
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (bf16)0.0f;
+    v[0] = f2bf(in[(b * 2 + 0) * n3 + q]);
+    v[1] = f2bf(in[(b * 2 + 1) * n3 + q]);
+    *reinterpret_cast<bf16x8 *>(X + i * 8) = v;
+}
+
+// 1x1 head 16 -> 1: out[p] = sum_c w[c] X[p][c] + w[16] (FP32 out)
+__global__ void head_fwd_kernel(const bf16 *__restrict__ X, const float *__restrict__ w, float *__restrict__ out,
+                                size_t npts) {
+    const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    float a = w[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a += w[c] * bf2f(X[p * 16 + c]);
+    out[p] = a;
+}
+
+// loss = mean((pred - target)^2); dpred = 2 (pred - target) / N  (partials per block)
+__global__ void mse_kernel(const float *__restrict__ pred, const float *__restrict__ target, float *__restrict__ dpred,
+                           double *__restrict__ lpart, size_t npts) {
+    __shared__ double sh[256];
+    const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (p < npts) {
+        const float d = pred[p] - target[p];
+        dpred[p] = 2.0f * d / (float)npts;
+        l = (double)d * d;
+    }
+    sh[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
+}
+
+// head backward: dX[p][c] = dpred[p] w[c] (BF16); per-block partial grads of w (16) and b (1)
+__global__ void head_bwd_kernel(const bf16 *__restrict__ X, const float *__restrict__ w,
+                                const float *__restrict__ dpred, bf16 *__restrict__ dX, float *__restrict__ gpart,
+                                size_t npts) {
+    __shared__ float sh[17][256];
+    const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    float gw[17];
+#pragma unroll
+    for (int c = 0; c < 17; ++c) gw[c] = 0.f;
+    if (p < npts) {
+        const float d = dpred[p];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            dX[p * 16 + c] = f2bf(d * w[c]);
+            gw[c] = d * bf2f(X[p * 16 + c]);
+        }
+        gw[16] = d;
+    }
+#pragma unroll
+    for (int c = 0; c < 17; ++c) sh[c][threadIdx.x] = gw[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+#pragma unroll
+            for (int c = 0; c < 17; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 17) gpart[(size_t)blockIdx.x * 17 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Adam on FP32 master parameters
+__global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, size_t count, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 master conv weights [Cout][Cin_real][27] -> forward layout [Cout][Kp] (k = tap*Cin + ci)
+// and backward-data layout [Cin][Kp_t] (k = tap'*Cout + co, tap' = 26 - tap), BF16
+__global__ void pack_weights_kernel(const float *__restrict__ w, int Cout, int Cin_real, int Cin, int Kp,
+                                    int Kp_t, bf16 *__restrict__ Wf, bf16 *__restrict__ Wt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = Cout * Cin * 27;
+    if (i >= total) return;
+    const int tap = i % 27, ci = (i / 27) % Cin, co = i / (27 * Cin);
+    const float v = ci < Cin_real ? w[((size_t)co * Cin_real + ci) * 27 + tap] : 0.f;
+    Wf[(size_t)co * Kp + tap * Cin + ci] = f2bf(v);
+    if (Wt) Wt[(size_t)ci * Kp_t + (26 - tap) * Cout + co] = f2bf(v);
+}
+
+// weight-gradient layout [Cout][Kp] (k = tap*Cin + ci) -> master order [Cout][Cin_real][27]
+__global__ void unpack_wgrad_kernel(const float *__restrict__ gk, int Cout, int Cin_real, int Cin, int Kp,
+                                    float *__restrict__ g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Cout * Cin_real * 27) return;
+    const int tap = i % 27, ci = (i / 27) % Cin_real, co = i / (27 * Cin_real);
+    g[i] = gk[(size_t)co * Kp + tap * Cin + ci];
+}
+
+}  // namespace unet
+}  // namespace cfm
+
+using namespace cfm;
+using namespace cfm::unet;
+
+namespace {
+
+// Layer table: 10 3x3x3 convolutions (+ReLU) and a 1x1 head.
+struct ConvSpec {
+    int cin_real, cin, cout, level;
+};
+constexpr int kNConv = 10;
+constexpr ConvSpec kConv[kNConv] = {
+    {2, 8, 16, 0},   {16, 16, 16, 0},  // encoder level 1 (input padded to 8 channels)
+    {16, 16, 32, 1}, {32, 32, 32, 1},  // encoder level 2
+    {32, 32, 64, 2}, {64, 64, 64, 2},  // bottleneck
+    {96, 96, 32, 1}, {32, 32, 32, 1},  // decoder level 2 (64 upsampled + 32 skip)
+    {48, 48, 16, 0}, {16, 16, 16, 0},  // decoder level 1 (32 upsampled + 16 skip)
+};
+constexpr int kHeadIn = 16;
+
+inline int pad32(int k) { return (k + 31) / 32 * 32; }
+
+size_t param_count() {
+    size_t c = 0;
+    for (const auto &s : kConv) c += (size_t)s.cout * s.cin_real * 27 + s.cout;
+    return c + kHeadIn + 1;
+}
+
+inline dim3 blocks_for(size_t count, int threads = 256) { return dim3((unsigned)((count + threads - 1) / threads)); }
+
+}  // namespace
+
+struct cfm_unet {
+    int n = 0, batch = 0;
+    size_t P[3] = {0, 0, 0};  // points per level (batch included)
+    hipStream_t stream = nullptr;
+    // parameters
+    float *master = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    size_t nparam = 0;
+    size_t woff[kNConv], boff[kNConv], head_off = 0;
+    bf16 *Wf[kNConv] = {}, *Wt[kNConv] = {};
+    int Kp[kNConv], Kpt[kNConv];
+    float *gk = nullptr;  // weight gradient in kernel layout (largest layer)
+    float *partial = nullptr;
+    size_t partial_cap = 0;
+    int slab = 4096;
+    // activations
+    bf16 *X0 = nullptr, *A0 = nullptr, *S1 = nullptr, *Q1 = nullptr, *A2 = nullptr, *S2 = nullptr, *Q2 = nullptr,
+         *A4 = nullptr, *A5 = nullptr, *U2 = nullptr, *A6 = nullptr, *A7 = nullptr, *U1 = nullptr, *A8 = nullptr,
+         *A9 = nullptr;
+    float *pred = nullptr, *target = nullptr, *dpred = nullptr, *in_f = nullptr;
+    double *lpart = nullptr;
+    float *hpart = nullptr;
+    // gradients of activations (BF16), sized for the largest of each level
+    bf16 *g1a = nullptr, *g1b = nullptr, *g1c = nullptr, *g1w = nullptr;  // level 1, up to 48 channels
+    bf16 *g2a = nullptr, *g2b = nullptr, *g2c = nullptr, *g2w = nullptr;  // level 2, up to 96 channels
+    bf16 *g3a = nullptr, *g3b = nullptr;                                   // level 3, 64 channels
+    long long step = 0;
+    std::vector<void *> allocs;
+};
+
+static cfm_status unet_alloc(cfm_unet *u, void **p, size_t bytes) {
+    if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) return CFM_ENOMEM;
+    u->allocs.push_back(*p);
+    return CFM_OK;
+}
+
+extern "C" void cfm_unet_destroy(cfm_unet *u) {
+    if (!u) return;
+    if (u->stream) (void)hipStreamSynchronize(u->stream);
+    for (void *p : u->allocs) (void)hipFree(p);
+    if (u->stream) (void)hipStreamDestroy(u->stream);
+    delete u;
+}
+
+extern "C" size_t cfm_unet_param_count(void) { return param_count(); }
+
+extern "C" cfm_status cfm_unet_create(int n, int batch, cfm_unet **out) {
+    if (!out) return CFM_EINVAL;
+    *out = nullptr;
+    // batch * n^3 <= 2^27 keeps the weight-gradient slab count (points / 4096) within a grid's y extent
+    if (n < 4 || n % 4 != 0 || batch < 1 || (long long)batch * n * n * n > (1LL << 27)) return CFM_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_EDEVICE;
+    auto *u = new (std::nothrow) cfm_unet();
+    if (!u) return CFM_ENOMEM;
+    auto fail = [&](cfm_status s) {
+        cfm_unet_destroy(u);
+        return s;
+    };
+    u->n = n;
+    u->batch = batch;
+    for (int l = 0; l < 3; ++l) {
+        const size_t m = (size_t)(n >> l);
+        u->P[l] = (size_t)batch * m * m * m;
+    }
+    if (hipStreamCreate(&u->stream) != hipSuccess) return fail(CFM_EDEVICE);
+    u->nparam = param_count();
+    size_t off = 0;
+    for (int i = 0; i < kNConv; ++i) {
+        u->woff[i] = off;
+        off += (size_t)kConv[i].cout * kConv[i].cin_real * 27;
+        u->boff[i] = off;
+        off += kConv[i].cout;
+        u->Kp[i] = pad32(27 * kConv[i].cin);
+        u->Kpt[i] = pad32(27 * kConv[i].cout);
+    }
+    u->head_off = off;
+    cfm_status st = CFM_OK;
+    auto A = [&](auto **p, size_t bytes) {
+        if (st == CFM_OK) st = unet_alloc(u, (void **)p, bytes);
+    };
+    A(&u->master, 4 * u->nparam);
+    A(&u->grad, 4 * u->nparam);
+    A(&u->adam_m, 4 * u->nparam);
+    A(&u->adam_v, 4 * u->nparam);
+    size_t max_wk = 0, max_partial = 0;
+    for (int i = 0; i < kNConv; ++i) {
+        A(&u->Wf[i], 2 * (size_t)kConv[i].cout * u->Kp[i]);
+        if (i > 0) A(&u->Wt[i], 2 * (size_t)kConv[i].cin * u->Kpt[i]);
+        max_wk = std::max(max_wk, (size_t)kConv[i].cout * u->Kp[i]);
+        const size_t nslab = (u->P[kConv[i].level] + u->slab - 1) / u->slab;
+        max_partial = std::max(max_partial, nslab * kConv[i].cout * u->Kp[i]);
+    }
+    A(&u->gk, 4 * max_wk);
+    A(&u->partial, 4 * max_partial);
+    u->partial_cap = max_partial;
+    const size_t P1 = u->P[0], P2 = u->P[1], P3 = u->P[2];
+    A(&u->X0, 2 * P1 * 8);
+    A(&u->A0, 2 * P1 * 16);
+    A(&u->S1, 2 * P1 * 16);
+    A(&u->Q1, 2 * P2 * 16);
+    A(&u->A2, 2 * P2 * 32);
+    A(&u->S2, 2 * P2 * 32);
+    A(&u->Q2, 2 * P3 * 32);
+    A(&u->A4, 2 * P3 * 64);
+    A(&u->A5, 2 * P3 * 64);
+    A(&u->U2, 2 * P2 * 96);
+    A(&u->A6, 2 * P2 * 32);
+    A(&u->A7, 2 * P2 * 32);
+    A(&u->U1, 2 * P1 * 48);
+    A(&u->A8, 2 * P1 * 16);
+    A(&u->A9, 2 * P1 * 16);
+    A(&u->pred, 4 * P1);
+    A(&u->target, 4 * P1);
+    A(&u->dpred, 4 * P1);
+    A(&u->in_f, 4 * P1 * 2);
+    A(&u->lpart, 8 * ((P1 + 255) / 256));
+    A(&u->hpart, 4 * 17 * ((P1 + 255) / 256));
+    for (bf16 **g : {&u->g1a, &u->g1b, &u->g1c, &u->g1w}) A(g, 2 * P1 * 48);
+    for (bf16 **g : {&u->g2a, &u->g2b, &u->g2c, &u->g2w}) A(g, 2 * P2 * 96);
+    for (bf16 **g : {&u->g3a, &u->g3b}) A(g, 2 * P3 * 64);
+    if (st != CFM_OK) return fail(st);
+    if (hipMemset(u->master, 0, 4 * u->nparam) != hipSuccess || hipMemset(u->adam_m, 0, 4 * u->nparam) != hipSuccess ||
+        hipMemset(u->adam_v, 0, 4 * u->nparam) != hipSuccess)
+        return fail(CFM_EDEVICE);
+    *out = u;
+    return CFM_OK;
+}
+
+static cfm_status pack_all_weights(cfm_unet *u) {
+    for (int i = 0; i < kNConv; ++i) {
+        const ConvSpec &s = kConv[i];
+        CFM_HIP(hipMemsetAsync(u->Wf[i], 0, 2 * (size_t)s.cout * u->Kp[i], u->stream));
+        if (u->Wt[i]) CFM_HIP(hipMemsetAsync(u->Wt[i], 0, 2 * (size_t)s.cin * u->Kpt[i], u->stream));
+        const int total = s.cout * s.cin * 27;
+        hipLaunchKernelGGL(pack_weights_kernel, blocks_for(total), dim3(256), 0, u->stream, u->master + u->woff[i],
+                           s.cout, s.cin_real, s.cin, u->Kp[i], u->Kpt[i], u->Wf[i], u->Wt[i]);
+        CFM_HIP(hipGetLastError());
+    }
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_unet_set_params(cfm_unet *u, const float *params) {
+    if (!u || !params) return CFM_EINVAL;
+    CFM_HIP(hipMemcpy(u->master, params, 4 * u->nparam, hipMemcpyHostToDevice));
+    CFM_HIP(hipMemset(u->adam_m, 0, 4 * u->nparam));
+    CFM_HIP(hipMemset(u->adam_v, 0, 4 * u->nparam));
+    u->step = 0;
+    cfm_status st = pack_all_weights(u);
+    if (st != CFM_OK) return st;
+    CFM_HIP(hipStreamSynchronize(u->stream));
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
+    if (!u || !params) return CFM_EINVAL;
+    CFM_HIP(hipMemcpy(params, u->master, 4 * u->nparam, hipMemcpyDeviceToHost));
+    return CFM_OK;
+}
+
+// Y = conv(X); mt = Cout/16
+static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, int kp, const float *bias, bf16 *Y,
+                           int cout, int level, int relu) {
+    const int n = u->n >> level;
+    const size_t npts = u->P[level];
+    const size_t tiles = (npts + 15) / 16;
+    const dim3 grid((unsigned)((tiles + 3) / 4));
+    switch (cout / 16) {
+        case 1: hipLaunchKernelGGL(conv3_fwd_kernel<1>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
+        case 2: hipLaunchKernelGGL(conv3_fwd_kernel<2>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
+        case 3: hipLaunchKernelGGL(conv3_fwd_kernel<3>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
+        case 4: hipLaunchKernelGGL(conv3_fwd_kernel<4>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
+        case 6: hipLaunchKernelGGL(conv3_fwd_kernel<6>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
+        default: return CFM_EINVAL;
+    }
+    CFM_HIP(hipGetLastError());
+    return CFM_OK;
+}
+
+static cfm_status run_forward(cfm_unet *u, const float *input_host) {
+    const size_t P1 = u->P[0], P2 = u->P[1];
+    const int n = u->n, n3 = n * n * n;
+    CFM_HIP(hipMemcpyAsync(u->in_f, input_host, 4 * P1 * 2, hipMemcpyHostToDevice, u->stream));
+    hipLaunchKernelGGL(pack_input_kernel, blocks_for(P1), dim3(256), 0, u->stream, u->in_f, u->X0, n3, u->batch);
+    CFM_HIP(hipGetLastError());
+    const float *b[kNConv];
+    for (int i = 0; i < kNConv; ++i) b[i] = u->master + u->boff[i];
+    cfm_status st;
+#define CONV(I, X, Y) \
+    if ((st = conv_fwd(u, X, kConv[I].cin, u->Wf[I], u->Kp[I], b[I], Y, kConv[I].cout, kConv[I].level, 1)) != CFM_OK) return st
+    CONV(0, u->X0, u->A0);
+    CONV(1, u->A0, u->S1);
+    hipLaunchKernelGGL(pool_fwd_kernel, blocks_for(P2 * 16), dim3(256), 0, u->stream, u->S1, u->Q1, 16, n, u->batch);
+    CONV(2, u->Q1, u->A2);
+    CONV(3, u->A2, u->S2);
+    hipLaunchKernelGGL(pool_fwd_kernel, blocks_for(u->P[2] * 32), dim3(256), 0, u->stream, u->S2, u->Q2, 32, n / 2, u->batch);
+    CONV(4, u->Q2, u->A4);
+    CONV(5, u->A4, u->A5);
+    hipLaunchKernelGGL(up_fwd_kernel, blocks_for(P2 * 64), dim3(256), 0, u->stream, u->A5, u->U2, 64, 96, 0, n / 4, u->batch);
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->S2, 32, 0, u->U2, 96, 64, 32, P2);
+    CONV(6, u->U2, u->A6);
+    CONV(7, u->A6, u->A7);
+    hipLaunchKernelGGL(up_fwd_kernel, blocks_for(P1 * 32), dim3(256), 0, u->stream, u->A7, u->U1, 32, 48, 0, n / 2, u->batch);
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->S1, 16, 0, u->U1, 48, 32, 16, P1);
+    CONV(8, u->U1, u->A8);
+    CONV(9, u->A8, u->A9);
+#undef CONV
+    hipLaunchKernelGGL(head_fwd_kernel, blocks_for(P1), dim3(256), 0, u->stream, u->A9, u->master + u->head_off,
+                       u->pred, P1);
+    CFM_HIP(hipGetLastError());
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_unet_forward(cfm_unet *u, const float *input, float *output) {
+    if (!u || !input || !output) return CFM_EINVAL;
+    cfm_status st = run_forward(u, input);
+    if (st != CFM_OK) return st;
+    CFM_HIP(hipMemcpyAsync(output, u->pred, 4 * u->P[0], hipMemcpyDeviceToHost, u->stream));
+    CFM_HIP(hipStreamSynchronize(u->stream));
+    return CFM_OK;
+}
+
+// weight and bias gradients of conv I from its input X and output gradient dZ
+static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) {
+    const ConvSpec &s = kConv[I];
+    const int level = s.level, n = u->n >> level;
+    const size_t npts = u->P[level];
+    const int nslab = (int)((npts + u->slab - 1) / u->slab);
+    const int ntile_k = u->Kp[I] / 16;
+    const int ntile = (s.cout / 16) * ntile_k;
+    const size_t count = (size_t)s.cout * u->Kp[I];
+    hipLaunchKernelGGL(conv3_wgrad_kernel, dim3((ntile + 3) / 4, nslab), dim3(256), 0, u->stream, X, s.cin, dZ, s.cout,
+                       u->Kp[I], u->partial, n, (int)npts, u->slab, ntile_k);
+    CFM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->partial, nslab, count, u->gk);
+    hipLaunchKernelGGL(unpack_wgrad_kernel, blocks_for((size_t)s.cout * s.cin_real * 27), dim3(256), 0, u->stream,
+                       u->gk, s.cout, s.cin_real, s.cin, u->Kp[I], u->grad + u->woff[I]);
+    hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(64), 0, u->stream, dZ, s.cout, (int)npts, u->slab, u->partial);
+    hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(s.cout), dim3(256), 0, u->stream, u->partial, nslab,
+                       (size_t)s.cout, u->grad + u->boff[I]);
+    CFM_HIP(hipGetLastError());
+    return CFM_OK;
+}
+
+// dX = conv_transpose(dZ) for conv I (output channels = cin of the layer)
+static cfm_status conv_dgrad(cfm_unet *u, int I, const bf16 *dZ, bf16 *dX) {
+    const ConvSpec &s = kConv[I];
+    return conv_fwd(u, dZ, s.cout, u->Wt[I], u->Kpt[I], nullptr, dX, s.cin, s.level, 0);
+}
+
+static cfm_status relu_bwd(cfm_unet *u, const bf16 *Y, const bf16 *dY, bf16 *dZ, size_t count) {
+    hipLaunchKernelGGL(relu_bwd_kernel, blocks_for(count), dim3(256), 0, u->stream, Y, dY, dZ, count);
+    CFM_HIP(hipGetLastError());
+    return CFM_OK;
+}
+
+static cfm_status run_backward(cfm_unet *u, const float *target_host, double *loss) {
+    const size_t P1 = u->P[0], P2 = u->P[1], P3 = u->P[2];
+    const int n = u->n;
+    CFM_HIP(hipMemcpyAsync(u->target, target_host, 4 * P1, hipMemcpyHostToDevice, u->stream));
+    const unsigned nb1 = (unsigned)((P1 + 255) / 256);
+    hipLaunchKernelGGL(mse_kernel, dim3(nb1), dim3(256), 0, u->stream, u->pred, u->target, u->dpred, u->lpart, P1);
+    const float *hw = u->master + u->head_off;
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(nb1), dim3(256), 0, u->stream, u->A9, hw, u->dpred, u->g1a, u->hpart, P1);
+    CFM_HIP(hipGetLastError());
+    // head gradient: reduce per-block partials (fixed order)
+    hipLaunchKernelGGL(reduce_slabs_kernel, dim3(1), dim3(64), 0, u->stream, u->hpart, (int)nb1, (size_t)17,
+                       u->grad + u->head_off);
+    cfm_status st;
+#define CK(x) \
+    if ((st = (x)) != CFM_OK) return st
+    // decoder level 1
+    CK(relu_bwd(u, u->A9, u->g1a, u->g1b, P1 * 16));   // dZ9
+    CK(conv_wgrad(u, 9, u->A8, u->g1b));
+    CK(conv_dgrad(u, 9, u->g1b, u->g1a));               // dA8
+    CK(relu_bwd(u, u->A8, u->g1a, u->g1b, P1 * 16));   // dZ8
+    CK(conv_wgrad(u, 8, u->U1, u->g1b));
+    CK(conv_dgrad(u, 8, u->g1b, u->g1c));               // dU1 [P1][48]
+    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g1c, u->g2a, 32, 48, 0, n / 2, u->batch);  // dA7
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->g1c, 48, 32, u->g1w, 16, 0, 16, P1);  // dS1 (skip)
+    // decoder level 2
+    CK(relu_bwd(u, u->A7, u->g2a, u->g2b, P2 * 32));   // dZ7
+    CK(conv_wgrad(u, 7, u->A6, u->g2b));
+    CK(conv_dgrad(u, 7, u->g2b, u->g2a));               // dA6
+    CK(relu_bwd(u, u->A6, u->g2a, u->g2b, P2 * 32));   // dZ6
+    CK(conv_wgrad(u, 6, u->U2, u->g2b));
+    CK(conv_dgrad(u, 6, u->g2b, u->g2c));               // dU2 [P2][96]
+    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P3 * 64), dim3(256), 0, u->stream, u->g2c, u->g3a, 64, 96, 0, n / 4, u->batch);  // dA5
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g2c, 96, 64, u->g2w, 32, 0, 32, P2);  // dS2 (skip)
+    // bottleneck
+    CK(relu_bwd(u, u->A5, u->g3a, u->g3b, P3 * 64));   // dZ5
+    CK(conv_wgrad(u, 5, u->A4, u->g3b));
+    CK(conv_dgrad(u, 5, u->g3b, u->g3a));               // dA4
+    CK(relu_bwd(u, u->A4, u->g3a, u->g3b, P3 * 64));   // dZ4
+    CK(conv_wgrad(u, 4, u->Q2, u->g3b));
+    CK(conv_dgrad(u, 4, u->g3b, u->g3a));               // dQ2 [P3][32]
+    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g3a, u->g2a, 32, n / 2, u->batch);
+    hipLaunchKernelGGL(add_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g2a, u->g2w, P2 * 32);  // dS2
+    // encoder level 2
+    CK(relu_bwd(u, u->S2, u->g2a, u->g2b, P2 * 32));   // dZ3
+    CK(conv_wgrad(u, 3, u->A2, u->g2b));
+    CK(conv_dgrad(u, 3, u->g2b, u->g2a));               // dA2
+    CK(relu_bwd(u, u->A2, u->g2a, u->g2b, P2 * 32));   // dZ2
+    CK(conv_wgrad(u, 2, u->Q1, u->g2b));
+    CK(conv_dgrad(u, 2, u->g2b, u->g2a));               // dQ1 [P2][16]
+    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->g2a, u->g1a, 16, n, u->batch);
+    hipLaunchKernelGGL(add_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->g1a, u->g1w, P1 * 16);  // dS1
+    // encoder level 1
+    CK(relu_bwd(u, u->S1, u->g1a, u->g1b, P1 * 16));   // dZ1
+    CK(conv_wgrad(u, 1, u->A0, u->g1b));
+    CK(conv_dgrad(u, 1, u->g1b, u->g1a));               // dA0
+    CK(relu_bwd(u, u->A0, u->g1a, u->g1b, P1 * 16));   // dZ0
+    CK(conv_wgrad(u, 0, u->X0, u->g1b));
+#undef CK
+    CFM_HIP(hipGetLastError());
+    if (loss) {
+        std::vector<double> lp(nb1);
+        CFM_HIP(hipMemcpyAsync(lp.data(), u->lpart, 8 * nb1, hipMemcpyDeviceToHost, u->stream));
+        CFM_HIP(hipStreamSynchronize(u->stream));
+        double s = 0.0;
+        for (double v : lp) s += v;
+        *loss = s / (double)P1;
+    }
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_unet_gradient(cfm_unet *u, const float *input, const float *target, float *grad,
+                                        double *loss) {
+    if (!u || !input || !target) return CFM_EINVAL;
+    cfm_status st = run_forward(u, input);
+    if (st == CFM_OK) st = run_backward(u, target, loss);
+    if (st != CFM_OK) return st;
+    if (grad) CFM_HIP(hipMemcpyAsync(grad, u->grad, 4 * u->nparam, hipMemcpyDeviceToHost, u->stream));
+    CFM_HIP(hipStreamSynchronize(u->stream));
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_unet_train_step(cfm_unet *u, const float *input, const float *target, float lr,
+                                          double *loss) {
+    if (!u || !input || !target || !(lr > 0.f)) return CFM_EINVAL;
+    cfm_status st = run_forward(u, input);
+    if (st == CFM_OK) st = run_backward(u, target, loss);
+    if (st != CFM_OK) return st;
+    u->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)u->step), c2 = 1.f - std::pow(b2, (float)u->step);
+    hipLaunchKernelGGL(adam_kernel, blocks_for(u->nparam), dim3(256), 0, u->stream, u->master, u->grad, u->adam_m,
+                       u->adam_v, u->nparam, lr, b1, b2, 1e-8f, c1, c2);
+    CFM_HIP(hipGetLastError());
+    st = pack_all_weights(u);
+    if (st != CFM_OK) return st;
+    CFM_HIP(hipStreamSynchronize(u->stream));
+    return CFM_OK;
+}
+
+// He-uniform initialisation from a SplitMix64 stream: w ~ U(-a, a),
+// a = sqrt(6 / fan_in); biases 0; head weights scaled by 0.1.
+extern "C" cfm_status cfm_unet_init_params(uint64_t seed, float *params) {
+    if (!params) return CFM_EINVAL;
+    uint64_t s = seed ^ 0xA0761D6478BD642FULL;
+    auto uni = [&]() {
+        s += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1.0p-53;
+    };
+    size_t off = 0;
+    for (const auto &c : kConv) {
+        const double a = std::sqrt(6.0 / (27.0 * c.cin_real));
+        const size_t nw = (size_t)c.cout * c.cin_real * 27;
+        for (size_t i = 0; i < nw; ++i) params[off + i] = (float)((2.0 * uni() - 1.0) * a);
+        off += nw;
+        for (int i = 0; i < c.cout; ++i) params[off + i] = 0.f;
+        off += c.cout;
+    }
+    const double a = 0.1 * std::sqrt(6.0 / kHeadIn);
+    for (int i = 0; i < kHeadIn; ++i) params[off + i] = (float)((2.0 * uni() - 1.0) * a);
+    params[off + kHeadIn] = 0.f;
+    return CFM_OK;
+}

@@ -225,6 +225,43 @@ int64_t cfm_gen_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int 
 /* SplitMix64 stream (seed, stream_id) -> count doubles in [0,1). */
 cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count);
 
+/* ---------------------------------------------------- learned warm start --- */
+/* No counterpart in the reference: a 3-level 3D U-Net that maps (f, log k)
+ * on an n^3 grid to an initial guess u0 for the CG solvers above, trained
+ * and run by hand-written BF16 MFMA kernels (FP32 accumulate).
+ *
+ * Architecture (3x3x3 convolutions, zero padding, ReLU after each):
+ *   enc1: 2->16, 16->16        pool 2x2x2 (average)
+ *   enc2: 16->32, 32->32       pool
+ *   mid : 32->64, 64->64       nearest upsample, concat [up, enc2]
+ *   dec2: 96->32, 32->32       nearest upsample, concat [up, enc1]
+ *   dec1: 48->16, 16->16       head: 1x1x1 16->1 (FP32, no ReLU)
+ * Flat FP32 parameter vector, in layer order: per convolution
+ * W[Cout][Cin][27] (tap = 9(dz+1) + 3(dy+1) + (dx+1)) then b[Cout]; then
+ * head w[16], head b.  Master parameters stay FP32; forward and backward use
+ * BF16 copies of the weights and BF16 activations.
+ * input: float [batch][2][n^3] (channel 0 = f, 1 = log k; index (z*n+y)*n+x);
+ * output/target: float [batch][n^3].  Loss = mean squared error. */
+typedef struct cfm_unet cfm_unet;
+size_t cfm_unet_param_count(void);
+/* n: multiple of 4, >= 4; batch * n^3 <= 2^27. */
+cfm_status cfm_unet_create(int n, int batch, cfm_unet **out);
+void cfm_unet_destroy(cfm_unet *net);
+/* He-uniform weights (U(-a, a), a = sqrt(6 / (27 Cin))), zero biases, head
+ * scaled by 0.1, from a SplitMix64 stream of `seed` (host only). */
+cfm_status cfm_unet_init_params(uint64_t seed, float *params);
+/* Uploads parameters and resets the Adam state. */
+cfm_status cfm_unet_set_params(cfm_unet *net, const float *params);
+cfm_status cfm_unet_get_params(cfm_unet *net, float *params);
+cfm_status cfm_unet_forward(cfm_unet *net, const float *input, float *output);
+/* Forward + backward; grad (may be NULL) receives d loss / d params in the
+ * flat order; loss (may be NULL) the mean squared error. */
+cfm_status cfm_unet_gradient(cfm_unet *net, const float *input, const float *target, float *grad,
+                             double *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr. */
+cfm_status cfm_unet_train_step(cfm_unet *net, const float *input, const float *target, float lr,
+                               double *loss);
+
 #ifdef __cplusplus
 }
 #endif

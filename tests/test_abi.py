@@ -100,3 +100,27 @@ def test_block_diag_layout(cfm):
     assert srp.tolist() == [0, 2, 4]
     assert rp.tolist() == [0, 1, 2, 4, 6]
     assert col.tolist() == [0, 1, 2, 3, 2, 3]
+
+
+def test_unet_host_side(cfm):
+    assert cfm.unet_param_count() == sum(co * ci * 27 + co for ci, co in cfm.UNET_CONVS) + 17
+    p = cfm.unet_init_params(5)
+    assert np.array_equal(p, cfm.unet_init_params(5))
+    assert not np.array_equal(p, cfm.unet_init_params(6))
+    off = 0
+    for ci, co in cfm.UNET_CONVS:
+        a = np.sqrt(6.0 / (27 * ci))
+        w = p[off: off + co * ci * 27]
+        assert np.all(np.abs(w) <= a) and np.abs(w).max() > 0.9 * a
+        assert abs(float(w.mean())) < 0.1 * a
+        off += co * ci * 27
+        assert np.all(p[off: off + co] == 0)
+        off += co
+    assert np.all(np.abs(p[off: off + 16]) <= 0.1 * np.sqrt(6.0 / 16)) and p[off + 16] == 0
+
+
+@pytest.mark.parametrize("n,batch", [(6, 1), (0, 1), (8, 0), (2048, 64)])
+def test_unet_create_rejects_bad_shapes(cfm, n, batch):
+    with pytest.raises(cfm.CfmError) as e:
+        cfm.UNet(n, batch)
+    assert e.value.status == cfm.EINVAL

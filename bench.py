@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--no-warmstart", action="store_true")
+    ap.add_argument("--warmstart-systems", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 1 GiB streaming-copy probe (counter calibration) and exit")
@@ -129,6 +131,18 @@ def cg_batch_figures(cfm, nsys, lmin=32, lmax=96, seed=2024):
             "mean_iters": round(float(iters.mean()), 1), "tuned_block": blk, "tuned_grid": grid}
 
 
+def warmstart_figures(nsys):
+    """Learned warm start on held-out 64^3 variable-coefficient systems: one
+    batched plan solved from x0 = 0 and from the trained network's guess
+    (inference + 2 damped Jacobi sweeps + guard), both to 1e-8, plus plain
+    single-system GPU CG on a 32-system sample."""
+    import warmstart as ws
+    params, meta = ws.load()
+    r = ws.evaluate(params, meta, systems=nsys, group=nsys, log=lambda *_: None)
+    r["trained_on_systems"] = meta["systems"]
+    return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -200,6 +214,8 @@ def main():
         line["cg_128"] = cg_figures(cfm)
     if not args.no_batch:
         line["cg_batch"] = cg_batch_figures(cfm, args.batch_systems)
+    if not args.no_warmstart:
+        line["cg_warmstart"] = warmstart_figures(args.warmstart_systems)
     gbs, _ = cfm.bandwidth_probe(1 << 30, 5)
     line["roofline"]["copy_ceiling_gbs"] = round(gbs, 1)
     if not args.no_cpu:

@@ -39,6 +39,8 @@ EXPORTS = (
     "cfm_cg_batch_solve", "cfm_cg_batch_get_x", "cfm_cg_batch_tune", "cfm_cg_batch_plan_destroy",
     "cfm_unet_param_count", "cfm_unet_create", "cfm_unet_destroy", "cfm_unet_init_params", "cfm_unet_set_params",
     "cfm_unet_get_params", "cfm_unet_forward", "cfm_unet_gradient", "cfm_unet_train_step",
+    "cfm_cg_batch_x0_rejected", "cfm_cg_batch_load_features", "cfm_cg_batch_warm_start", "cfm_gen_varcoef3d_coef",
+    "cfm_cg_batch_smooth_x0",
 )
 
 
@@ -67,7 +69,7 @@ class CgParams(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("mode", ctypes.c_int), ("tol", ctypes.c_double),
                 ("max_iter", ctypes.c_int), ("restart_quirk", ctypes.c_int), ("max_outer", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("grid_blocks", ctypes.c_int),
-                ("spin_limit_us", ctypes.c_int), ("block_threads", ctypes.c_int)]
+                ("spin_limit_us", ctypes.c_int), ("block_threads", ctypes.c_int), ("x0_guard", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -133,6 +135,11 @@ def _load():
     L.cfm_unet_forward.argtypes = [_vp, _fp, _fp]
     L.cfm_unet_gradient.argtypes = [_vp, _fp, _fp, _fp, _dp]
     L.cfm_unet_train_step.argtypes = [_vp, _fp, _fp, ctypes.c_float, _dp]
+    L.cfm_cg_batch_x0_rejected.argtypes = [_vp, _ip]
+    L.cfm_cg_batch_load_features.argtypes = [_vp, _fp]
+    L.cfm_cg_batch_warm_start.argtypes = [_vp, _vp, ctypes.c_double, ctypes.POINTER(Timing)]
+    L.cfm_gen_varcoef3d_coef.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
+    L.cfm_cg_batch_smooth_x0.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Timing)]
     return L
 
 
@@ -284,9 +291,9 @@ def uniform(seed, stream_id, count):
 
 
 def cg_params(method=CG_STANDARD, mode=CG_PERSISTENT, tol=1e-8, max_iter=10000, restart_quirk=1,
-              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0, block_threads=0):
+              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0, block_threads=0, x0_guard=0):
     return CgParams(method, mode, tol, max_iter, restart_quirk, max_outer, inner_tol, grid_blocks,
-                    spin_limit_us, block_threads)
+                    spin_limit_us, block_threads, x0_guard)
 
 
 class CgPlan:
@@ -383,6 +390,18 @@ def varcoef3d(seed, system_id, lmin, lmax):
     return rowptr, col, val, f, tuple(int(d) for d in dims)
 
 
+def varcoef3d_coef(seed, system_id, lmin, lmax):
+    """Per-cell coefficients k of the varcoef3d system, shape (Lz, Ly, Lx)."""
+    dims = np.zeros(3, dtype=np.int32)
+    nnz = lib.cfm_gen_varcoef3d(seed, system_id, lmin, lmax, dims.ctypes.data_as(_ip), None, None, None, None)
+    if nnz < 0:
+        raise CfmError(EINVAL, "cfm_gen_varcoef3d_coef")
+    k = np.empty(int(dims.prod()))
+    _check(lib.cfm_gen_varcoef3d_coef(seed, system_id, lmin, lmax, dims.ctypes.data_as(_ip), k.ctypes.data_as(_dp)),
+           "cfm_gen_varcoef3d_coef")
+    return k.reshape(int(dims[2]), int(dims[1]), int(dims[0]))
+
+
 def block_diag(systems):
     """Concatenate [(rowptr, col, val), ...] into one block-diagonal CSR.
     Returns (sys_row_ptr int64, rowptr, col, val)."""
@@ -447,6 +466,30 @@ class CgBatchPlan:
         out = np.empty(self.n)
         _check(lib.cfm_cg_batch_get_x(self._h, out.ctypes.data_as(_dp)), "cfm_cg_batch_get_x")
         return out
+
+    def x0_rejected(self):
+        out = np.zeros(self.nsys, dtype=np.int32)
+        _check(lib.cfm_cg_batch_x0_rejected(self._h, out.ctypes.data_as(_ip)), "cfm_cg_batch_x0_rejected")
+        return out
+
+    def load_features(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        if f.size != 2 * self.n:
+            raise ValueError("features must hold 2 values per row")
+        _check(lib.cfm_cg_batch_load_features(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+               "cfm_cg_batch_load_features")
+
+    def warm_start(self, net, scale):
+        """x0 := scale * net(features) on the device; returns the timing dict."""
+        t = Timing()
+        _check(lib.cfm_cg_batch_warm_start(self._h, net._h, float(scale), ctypes.byref(t)), "cfm_cg_batch_warm_start")
+        return t.as_dict()
+
+    def smooth_x0(self, sweeps, omega=0.8):
+        """Damped Jacobi sweeps on x0 (device); returns the timing dict."""
+        t = Timing()
+        _check(lib.cfm_cg_batch_smooth_x0(self._h, int(sweeps), float(omega), ctypes.byref(t)), "cfm_cg_batch_smooth_x0")
+        return t.as_dict()
 
     def tune(self, trial_iters=20, cache_path=None):
         bb, bg = ctypes.c_int(0), ctypes.c_int(0)

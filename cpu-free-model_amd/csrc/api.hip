@@ -187,36 +187,62 @@ struct SplitMix {
 };
 }  // namespace
 
-extern "C" int64_t cfm_gen_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr,
-                                     int *col, double *val, double *f) {
-    if (!dims || lmin < 1 || lmax < lmin || system_id < 0) return -1;
+// Draws of one variable-coefficient system: box lengths, then k per cell,
+// then f per cell, all from the SplitMix64 stream (seed, system_id).
+static bool varcoef_draw(uint64_t seed, int system_id, int lmin, int lmax, int L[3], std::vector<double> *k,
+                         double *f) {
+    if (lmin < 1 || lmax < lmin || system_id < 0) return false;
     SplitMix g{seed ^ (0x9E3779B97F4A7C15ULL * ((uint64_t)system_id + 1))};
-    int L[3];
     for (int a = 0; a < 3; ++a) {
         const double u = g.next();
         int v = (int)std::floor(std::exp(std::log((double)lmin) + u * (std::log((double)lmax + 1.0) - std::log((double)lmin))));
         L[a] = v < lmin ? lmin : v > lmax ? lmax : v;
-        dims[a] = L[a];
     }
-    const long long nx = L[0], ny = L[1], nz = L[2], n = nx * ny * nz, plane = nx * ny;
-    if (n * 7 > 0x7fffffffLL) return -1;
-    long long nnz = 0;
-    for (long long i = 0; i < n; ++i) {
-        const long long ix = i % nx, iy = (i / nx) % ny, iz = i / plane;
-        nnz += 1 + (iz > 0) + (iy > 0) + (ix > 0) + (ix + 1 < nx) + (iy + 1 < ny) + (iz + 1 < nz);
-    }
-    if (!rowptr && !col && !val && !f) return nnz;
-    std::vector<double> k((size_t)n);
+    const long long n = (long long)L[0] * L[1] * L[2];
+    if (n * 7 > 0x7fffffffLL) return false;
+    if (!k) return true;
+    k->resize((size_t)n);
     for (long long i = 0; i < n; i += 2) {
         const double u1 = g.next(), u2 = g.next();
         const double r = std::sqrt(-2.0 * std::log(1.0 - u1));
         const double th = 2.0 * 3.14159265358979323846 * u2;
-        k[i] = std::exp(r * host_cos(th));
-        if (i + 1 < n) k[i + 1] = std::exp(r * host_sin(th));
+        (*k)[i] = std::exp(r * host_cos(th));
+        if (i + 1 < n) (*k)[i + 1] = std::exp(r * host_sin(th));
     }
     for (long long i = 0; i < n; ++i) {
         const double u = g.next();
         if (f) f[i] = 2.0 * u - 1.0;
+    }
+    return true;
+}
+
+extern "C" cfm_status cfm_gen_varcoef3d_coef(uint64_t seed, int system_id, int lmin, int lmax, int *dims,
+                                             double *kcell) {
+    if (!dims || !kcell) return CFM_EINVAL;
+    int L[3];
+    std::vector<double> k;
+    if (!varcoef_draw(seed, system_id, lmin, lmax, L, &k, nullptr)) return CFM_EINVAL;
+    for (int a = 0; a < 3; ++a) dims[a] = L[a];
+    std::memcpy(kcell, k.data(), sizeof(double) * k.size());
+    return CFM_OK;
+}
+
+extern "C" int64_t cfm_gen_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr,
+                                     int *col, double *val, double *f) {
+    if (!dims) return -1;
+    int L[3];
+    const bool sizes_only = !rowptr && !col && !val && !f;
+    std::vector<double> k;
+    if (!varcoef_draw(seed, system_id, lmin, lmax, L, sizes_only ? nullptr : &k, f)) return -1;
+    for (int a = 0; a < 3; ++a) dims[a] = L[a];
+    const long long nx = L[0], ny = L[1], nz = L[2], n = nx * ny * nz, plane = nx * ny;
+    if (sizes_only) {
+        long long nnz = 0;
+        for (long long i = 0; i < n; ++i) {
+            const long long ix = i % nx, iy = (i / nx) % ny, iz = i / plane;
+            nnz += 1 + (iz > 0) + (iy > 0) + (ix > 0) + (ix + 1 < nx) + (iy + 1 < ny) + (iz + 1 < nz);
+        }
+        return nnz;
     }
     long long e = 0;
     for (long long i = 0; i < n; ++i) {

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "cg_common.hpp"
+#include "unet.hpp"
 
 namespace cfm {
 
@@ -56,6 +57,8 @@ struct BatchArgs {
     int max_iter;
     double tol;
     int quirk;
+    int x0_guard;      // restart a system from x0 = 0 when <r0,r0> > <b,b>
+    int *x0_rejected;  // [nsys] 1 where the guard restarted the system
     GridSync *sync;
 };
 
@@ -130,7 +133,7 @@ __global__ void __launch_bounds__(256 * SUB) cg_batch_kernel(BatchArgs A) {
     const int sub = threadIdx.x >> 8;
     unsigned int epoch = 0;
 
-    // ---- init 1: r = b - A x0, x = x0, z = s = p = 0, partial <b,b> (all chunks)
+    // ---- init 1: r = b - A x0, x = x0, z = s = p = 0, partials <b,b>, <r0,r0> (all chunks)
     for (int base = blockIdx.x * SUB; base < A.total_chunks; base += gridDim.x * SUB) {
         const int c = base + sub;
         const bool act = c < A.total_chunks;
@@ -140,7 +143,7 @@ __global__ void __launch_bounds__(256 * SUB) cg_batch_kernel(BatchArgs A) {
             row0 = A.sys_chunk0[sy] * kChunk;
             nrows = A.sys_rows[sy];
         }
-        double ab = 0.0;
+        double ab = 0.0, ar = 0.0;
         if (act) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -149,20 +152,30 @@ __global__ void __launch_bounds__(256 * SUB) cg_batch_kernel(BatchArgs A) {
                 if (row - row0 < nrows) {
                     const double ax = sell_row<double, double>(A.A, A.x0, slice, t & 63);
                     const double bi = A.b[row];
-                    A.r[row] = fma(-1.0, ax, bi);
+                    const double ri = fma(-1.0, ax, bi);
+                    A.r[row] = ri;
                     A.x[row] = A.x0[row];
                     A.z[row] = 0.0;
                     A.s[row] = 0.0;
                     A.p[row] = 0.0;
                     ab = fma(bi, bi, ab);
+                    ar = fma(ri, ri, ar);
                 }
             }
         }
         const double sb = sub_sum<SUB>(ab, S.red);
-        if (act && t == 0) A.part[4 * A.total_chunks + c] = sb;
+        const double sr = sub_sum<SUB>(ar, S.red);
+        if (act && t == 0) {
+            A.part[4 * A.total_chunks + c] = sb;
+            A.part[1 * A.total_chunks + c] = sr;  // delta buffer 1 is free until iteration 0 writes it
+        }
     }
     if (!grid_barrier(A.sync, epoch)) return;
-    // ---- init 2: w0 = A r, partials <r,r>, <r,w0> into buffer 0
+    // ---- init 2: guard decision, w0 = A r, partials <r,r>, <r,w0> into buffer 0.
+    // With the guard on, every chunk of a system reduces that system's <r0,r0>
+    // and <b,b> with the same tree, so all of them reach the same decision; a
+    // rejected system takes x = 0, r = b and w0 = A b (b is read-only, so no
+    // chunk reads a residual another chunk is rewriting).
     for (int base = blockIdx.x * SUB; base < A.total_chunks; base += gridDim.x * SUB) {
         const int c = base + sub;
         const bool act = c < A.total_chunks;
@@ -172,6 +185,21 @@ __global__ void __launch_bounds__(256 * SUB) cg_batch_kernel(BatchArgs A) {
             row0 = A.sys_chunk0[sy] * kChunk;
             nrows = A.sys_rows[sy];
         }
+        bool reject = false;
+        if (A.x0_guard) {  // uniform over the workgroup
+            double a1 = 0.0, a3 = 0.0;
+            if (act) {
+                const int c0 = A.sys_chunk0[sy], nc = A.sys_chunk0[sy + 1] - c0;
+                for (int j = t; j < nc; j += kThreads) {
+                    a1 = a1 + A.part[1 * A.total_chunks + c0 + j];
+                    a3 = a3 + A.part[4 * A.total_chunks + c0 + j];
+                }
+            }
+            const double d0 = sub_sum<SUB>(a1, S.red);
+            const double bb = sub_sum<SUB>(a3, S.red);
+            reject = act && !(d0 <= bb);  // a non-finite residual is rejected too
+            if (act && t == 0 && c == A.sys_chunk0[sy]) A.x0_rejected[sy] = reject ? 1 : 0;
+        }
         double ad = 0.0, ag = 0.0;
         if (act) {
 #pragma unroll
@@ -179,9 +207,17 @@ __global__ void __launch_bounds__(256 * SUB) cg_batch_kernel(BatchArgs A) {
                 const int slice = c * kSlicesPerChunk + i * 4 + (t >> 6);
                 const int row = c * kChunk + i * kThreads + t;
                 if (row - row0 < nrows) {
-                    const double wi = sell_row<double, double>(A.A, A.r, slice, t & 63);
+                    double wi, ri;
+                    if (reject) {
+                        wi = sell_row<double, double>(A.A, A.b, slice, t & 63);
+                        ri = A.b[row];
+                        A.r[row] = ri;
+                        A.x[row] = 0.0;
+                    } else {
+                        wi = sell_row<double, double>(A.A, A.r, slice, t & 63);
+                        ri = A.r[row];
+                    }
                     A.w[0][row] = wi;
-                    const double ri = A.r[row];
                     ad = fma(ri, ri, ad);
                     ag = fma(ri, wi, ag);
                 }
@@ -335,6 +371,44 @@ __global__ void __launch_bounds__(256 * SUB) cg_batch_kernel(BatchArgs A) {
     }
 }
 
+// x0[chunk-aligned row of system first + j / n3] = scale * pred[j], j < count * n3
+__global__ void scatter_x0_kernel(const float *__restrict__ pred, const int *__restrict__ sys_chunk0, int first,
+                                  int count, int n3, double scale, double *__restrict__ x0) {
+    const size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (j >= (size_t)count * n3) return;
+    const int q = (int)(j / n3), i = (int)(j % n3);
+    x0[(size_t)sys_chunk0[first + q] * kChunk + i] = scale * (double)pred[j];
+}
+
+// One damped Jacobi sweep: xo = x + omega * (b - A x) / a_ii.  (A x)_i is
+// the SpMV row sum (fma in stored column order), then r = fma(-1, Ax, b),
+// xo = fma(omega, r / a_ii, x).  Rows without a stored diagonal (the chunk
+// padding) keep x.
+__global__ void jacobi_smooth_kernel(Sell A, const double *__restrict__ b, const double *__restrict__ x,
+                                     double *__restrict__ xo, int nrows, double omega) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const int slice = row >> 6, lane = row & 63;
+    const int beg = A.slice_ptr[slice];
+    const int width = (A.slice_ptr[slice + 1] - beg) >> 6;
+    const double *val = static_cast<const double *>(A.val);
+    double ax = 0.0, d = 0.0;
+    for (int j = 0; j < width; ++j) {
+        const int c = A.col[beg + j * 64 + lane];
+        if (c >= 0) {
+            const double v = val[beg + j * 64 + lane];
+            ax = fma(v, x[c], ax);
+            if (c == row) d = v;
+        }
+    }
+    if (d != 0.0) {
+        const double r = fma(-1.0, ax, b[row]);
+        xo[row] = fma(omega, r / d, x[row]);
+    } else {
+        xo[row] = x[row];
+    }
+}
+
 }  // namespace cfm
 
 using namespace cfm;
@@ -352,7 +426,8 @@ struct cfm_cg_batch_plan {
     double *b = nullptr, *x0 = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *s = nullptr, *pv = nullptr;
     double *w[2] = {nullptr, nullptr};
     double *part = nullptr, *st_alpha = nullptr, *st_delta = nullptr, *hist = nullptr;
-    int *fin = nullptr, *status = nullptr, *iters = nullptr;
+    int *fin = nullptr, *status = nullptr, *iters = nullptr, *x0_rejected = nullptr;
+    float *features = nullptr;  // warm-start network input [nsys][2][rows], loaded on demand
     int hist_cap = 0;
     GridSync *sync = nullptr;
     hipStream_t stream = nullptr;
@@ -376,7 +451,7 @@ extern "C" void cfm_cg_batch_plan_destroy(cfm_cg_batch_plan *pl) {
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     void *ptrs[] = {pl->sys_chunk0, pl->sys_rows, pl->slice_ptr, pl->scol, pl->sval, pl->b, pl->x0, pl->x,
                     pl->r, pl->z, pl->s, pl->pv, pl->w[0], pl->w[1], pl->part, pl->st_alpha, pl->st_delta,
-                    pl->hist, pl->fin, pl->status, pl->iters, pl->sync};
+                    pl->hist, pl->fin, pl->status, pl->iters, pl->x0_rejected, pl->features, pl->sync};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
@@ -486,6 +561,7 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
               dalloc((void **)&pl->st_delta, sizeof(double) * 2 * nsys) &&
               dalloc((void **)&pl->status, sizeof(int) * nsys) && dalloc((void **)&pl->iters, sizeof(int) * nsys) &&
               dalloc((void **)&pl->fin, sizeof(int) * nsys) &&
+              dalloc((void **)&pl->x0_rejected, sizeof(int) * nsys) &&
               dalloc((void **)&pl->sync, sizeof(GridSync));
     double **vecs[] = {&pl->b, &pl->x0, &pl->x, &pl->r, &pl->z, &pl->s, &pl->pv, &pl->w[0], &pl->w[1]};
     for (double **v : vecs) ok = ok && dalloc((void **)v, sizeof(double) * nv);
@@ -571,8 +647,11 @@ static cfm_status batch_run(cfm_cg_batch_plan *pl, int max_iter, double tol, int
     A.max_iter = max_iter;
     A.tol = tol;
     A.quirk = pl->p.restart_quirk;
+    A.x0_guard = pl->p.x0_guard;
+    A.x0_rejected = pl->x0_rejected;
     A.sync = pl->sync;
     CFM_HIP(hipMemsetAsync(pl->status, 0, sizeof(int) * pl->nsys, pl->stream));
+    CFM_HIP(hipMemsetAsync(pl->x0_rejected, 0, sizeof(int) * pl->nsys, pl->stream));
     CFM_HIP(hipMemsetD32Async((hipDeviceptr_t)pl->fin, 0x7fffffff, pl->nsys, pl->stream));
     CFM_HIP(hipMemsetAsync(pl->iters, 0, sizeof(int) * pl->nsys, pl->stream));
     hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
@@ -645,6 +724,8 @@ extern "C" cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *pl, int trial_iters, 
     char key[512];
     std::snprintf(key, sizeof(key), "cg_batch|%s|cus=%d|nsys=%d|rows=%lld|chunks=%d", pl->dev.name, pl->dev.cus,
                   pl->nsys, pl->total_rows, pl->total_chunks);
+    for (char *q = key; *q; ++q)  // the cache line is whitespace-separated
+        if (*q == ' ' || *q == '\t') *q = '_';
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "r")) {
             char line[1024];
@@ -692,5 +773,89 @@ extern "C" cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *pl, int trial_iters, 
     }
     if (best_block) *best_block = 256 * bs;
     if (best_grid) *best_grid = bg;
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_batch_x0_rejected(cfm_cg_batch_plan *pl, int *flags) {
+    if (!pl || !flags) return CFM_EINVAL;
+    CFM_HIP(hipMemcpy(flags, pl->x0_rejected, sizeof(int) * pl->nsys, hipMemcpyDeviceToHost));
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_batch_load_features(cfm_cg_batch_plan *pl, const float *features) {
+    if (!pl || !features) return CFM_EINVAL;
+    if (!pl->features) CFM_HIP(hipMalloc((void **)&pl->features, sizeof(float) * 2 * (size_t)pl->total_rows));
+    CFM_HIP(hipMemcpy(pl->features, features, sizeof(float) * 2 * (size_t)pl->total_rows, hipMemcpyHostToDevice));
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_batch_warm_start(cfm_cg_batch_plan *pl, cfm_unet *net, double scale, cfm_timing *t) {
+    if (!pl || !net || !(std::isfinite(scale))) return CFM_EINVAL;
+    if (!pl->features) return CFM_EINVAL;
+    const auto w0 = std::chrono::steady_clock::now();
+    const UnetView nv = unet_view(net);
+    const long long n3 = (long long)nv.n * nv.n * nv.n;
+    for (int sy = 0; sy < pl->nsys; ++sy)
+        if (pl->h_rows[sy] != n3) return CFM_EINVAL;  // every system must be the network's n^3 cube
+    CFM_HIP(hipStreamSynchronize(pl->stream));
+    CFM_HIP(hipEventRecord(pl->ev0, nv.stream));
+    for (int g0 = 0; g0 < pl->nsys; g0 += nv.batch) {
+        const int cnt = std::min(nv.batch, pl->nsys - g0);
+        const float *src = pl->features + (size_t)g0 * 2 * n3;
+        if (cnt < nv.batch) {  // a short last group runs from the network's own buffer, zero padded
+            CFM_HIP(hipMemcpyAsync(nv.input, src, sizeof(float) * 2 * n3 * cnt, hipMemcpyDeviceToDevice, nv.stream));
+            CFM_HIP(hipMemsetAsync(nv.input + 2 * n3 * cnt, 0, sizeof(float) * 2 * n3 * (nv.batch - cnt), nv.stream));
+            src = nv.input;
+        }
+        const float *pred = nullptr;
+        cfm_status st = unet_forward_device(net, src, &pred);
+        if (st != CFM_OK) return st;
+        const size_t count = (size_t)cnt * n3;
+        hipLaunchKernelGGL(scatter_x0_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, nv.stream, pred,
+                           pl->sys_chunk0, g0, cnt, (int)n3, scale, pl->x0);
+        CFM_HIP(hipGetLastError());
+    }
+    CFM_HIP(hipEventRecord(pl->ev1, nv.stream));
+    CFM_HIP(hipEventSynchronize(pl->ev1));
+    float ms = 0.f;
+    CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+    if (t) {
+        t->kernel_ms = ms;
+        t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        t->grid_blocks = 0;
+        t->block_threads = 0;
+        t->iterations = (pl->nsys + nv.batch - 1) / nv.batch;
+        t->outer_iterations = 0;
+    }
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_cg_batch_smooth_x0(cfm_cg_batch_plan *pl, int sweeps, double omega, cfm_timing *t) {
+    if (!pl || sweeps < 0 || !(omega > 0.0 && omega < 2.0)) return CFM_EINVAL;
+    const auto w0 = std::chrono::steady_clock::now();
+    const int nrows = pl->total_chunks * kChunk;
+    const Sell A{pl->slice_ptr, pl->scol, pl->sval};
+    CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
+    double *src = pl->x0, *dst = pl->r;  // r is scratch until the next solve initialises it
+    for (int k = 0; k < sweeps; ++k) {
+        hipLaunchKernelGGL(jacobi_smooth_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, pl->stream, A,
+                           pl->b, src, dst, nrows, omega);
+        CFM_HIP(hipGetLastError());
+        std::swap(src, dst);
+    }
+    if (src != pl->x0)
+        CFM_HIP(hipMemcpyAsync(pl->x0, src, sizeof(double) * nrows, hipMemcpyDeviceToDevice, pl->stream));
+    CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
+    CFM_HIP(hipEventSynchronize(pl->ev1));
+    float ms = 0.f;
+    CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
+    if (t) {
+        t->kernel_ms = ms;
+        t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        t->grid_blocks = (nrows + 255) / 256;
+        t->block_threads = 256;
+        t->iterations = sweeps;
+        t->outer_iterations = 0;
+    }
     return CFM_OK;
 }

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "cfm_common.hpp"
+#include "unet.hpp"
 
 namespace cfm {
 namespace unet {
@@ -575,11 +576,10 @@ static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, i
     return CFM_OK;
 }
 
-static cfm_status run_forward(cfm_unet *u, const float *input_host) {
+static cfm_status run_forward_dev(cfm_unet *u, const float *d_in) {
     const size_t P1 = u->P[0], P2 = u->P[1];
     const int n = u->n, n3 = n * n * n;
-    CFM_HIP(hipMemcpyAsync(u->in_f, input_host, 4 * P1 * 2, hipMemcpyHostToDevice, u->stream));
-    hipLaunchKernelGGL(pack_input_kernel, blocks_for(P1), dim3(256), 0, u->stream, u->in_f, u->X0, n3, u->batch);
+    hipLaunchKernelGGL(pack_input_kernel, blocks_for(P1), dim3(256), 0, u->stream, d_in, u->X0, n3, u->batch);
     CFM_HIP(hipGetLastError());
     const float *b[kNConv];
     for (int i = 0; i < kNConv; ++i) b[i] = u->master + u->boff[i];
@@ -608,6 +608,20 @@ static cfm_status run_forward(cfm_unet *u, const float *input_host) {
     CFM_HIP(hipGetLastError());
     return CFM_OK;
 }
+
+static cfm_status run_forward(cfm_unet *u, const float *input_host) {
+    CFM_HIP(hipMemcpyAsync(u->in_f, input_host, 4 * u->P[0] * 2, hipMemcpyHostToDevice, u->stream));
+    return run_forward_dev(u, u->in_f);
+}
+
+namespace cfm {
+UnetView unet_view(cfm_unet *u) { return UnetView{u->n, u->batch, u->stream, u->in_f}; }
+cfm_status unet_forward_device(cfm_unet *u, const float *d_in, const float **d_pred) {
+    cfm_status st = run_forward_dev(u, d_in);
+    if (st == CFM_OK && d_pred) *d_pred = u->pred;
+    return st;
+}
+}  // namespace cfm
 
 extern "C" cfm_status cfm_unet_forward(cfm_unet *u, const float *input, float *output) {
     if (!u || !input || !output) return CFM_EINVAL;

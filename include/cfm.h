@@ -149,6 +149,8 @@ typedef struct {
     int grid_blocks;     /* 0 = auto */
     int spin_limit_us;   /* device barrier watchdog; 0 = default (10 s) */
     int block_threads;   /* 256, 512, 768 or 1024 (256-row sub-blocks per workgroup); 0 = auto */
+    int x0_guard;        /* batched plans: nonzero restarts a system from x0 = 0 when
+                            ||b - A x0|| > ||b|| (the guard for learned initial guesses) */
 } cfm_cg_params;
 
 /* Replaces SingleStreamStandard::init / SingleStreamPipelined::init and
@@ -204,6 +206,9 @@ cfm_status cfm_cg_batch_get_x(cfm_cg_batch_plan *plan, double *x);
 cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *plan, int trial_iters, const char *cache_path,
                              int *best_block, int *best_grid);
 void cfm_cg_batch_plan_destroy(cfm_cg_batch_plan *plan);
+/* After a solve with x0_guard set: flags[s] = 1 where system s was restarted
+ * from x0 = 0 because its initial guess was worse than zero. */
+cfm_status cfm_cg_batch_x0_rejected(cfm_cg_batch_plan *plan, int *flags);
 
 /* ---------------------------------------------------------- generators --- */
 /* 3D 7-point Poisson on an nx*ny*nz grid, Dirichlet, rows ix + nx*(iy + ny*iz),
@@ -222,6 +227,9 @@ int64_t cfm_gen_poisson3d(int nx, int ny, int nz, int *rowptr, int *col, double 
  * the sizes (dims is always written). */
 int64_t cfm_gen_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dims, int *rowptr,
                           int *col, double *val, double *f);
+/* The per-cell coefficients k of the same system (kcell holds Lx*Ly*Lz values). */
+cfm_status cfm_gen_varcoef3d_coef(uint64_t seed, int system_id, int lmin, int lmax, int *dims,
+                                  double *kcell);
 /* SplitMix64 stream (seed, stream_id) -> count doubles in [0,1). */
 cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count);
 
@@ -261,6 +269,18 @@ cfm_status cfm_unet_gradient(cfm_unet *net, const float *input, const float *tar
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) at learning rate lr. */
 cfm_status cfm_unet_train_step(cfm_unet *net, const float *input, const float *target, float lr,
                                double *loss);
+/* Network input of every system of a batched plan: float [nsys][2][rows_s],
+ * packed in the plan's system order (uploaded once, kept on the device). */
+cfm_status cfm_cg_batch_load_features(cfm_cg_batch_plan *plan, const float *features);
+/* x0 of every system := scale * net(features), computed and written on the
+ * device (no host round trip); every system must be an n^3 cube of the
+ * network's n.  Timing: kernel_ms covers the inference and the x0 writes. */
+cfm_status cfm_cg_batch_warm_start(cfm_cg_batch_plan *plan, cfm_unet *net, double scale, cfm_timing *t);
+/* `sweeps` damped Jacobi sweeps on every system's x0 with the plan's matrix:
+ * x <- x + omega (b - A x) / a_ii (0 < omega < 2; rows without a stored
+ * diagonal keep x).  Removes the high-frequency error of a learned guess,
+ * which inflates ||b - A x0|| (the guard's test) but costs CG little. */
+cfm_status cfm_cg_batch_smooth_x0(cfm_cg_batch_plan *plan, int sweeps, double omega, cfm_timing *t);
 
 #ifdef __cplusplus
 }

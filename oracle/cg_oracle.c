@@ -425,3 +425,32 @@ int64_t orc_varcoef3d(uint64_t seed, int system_id, int lmin, int lmax, int *dim
     free(k);
     return e;
 }
+
+/* Damped Jacobi sweeps on a guess (the warm-start smoother of the batched
+ * plan, which has no counterpart in the reference).  Row i: ax = SpMV row
+ * sum (orc_spmv order), r = fma(-1, ax, b_i), x_i' = fma(omega, r / a_ii, x_i);
+ * a row without a stored diagonal keeps x_i. */
+int orc_jacobi_smooth(const orc_csr *A, const double *b, double *x, int sweeps, double omega) {
+    const int n = A->n, base = A->base;
+    double *y = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!y) return ORC_ENOMEM;
+    for (int k = 0; k < sweeps; ++k) {
+        for (int i = 0; i < n; ++i) {
+            double ax = 0.0, d = 0.0;
+            for (int j = A->rowptr[i] - base; j < A->rowptr[i + 1] - base; ++j) {
+                const int c = A->col[j] - base;
+                ax = fma(A->val[j], x[c], ax);
+                if (c == i) d = A->val[j];
+            }
+            if (d != 0.0) {
+                const double r = fma(-1.0, ax, b[i]);
+                y[i] = fma(omega, r / d, x[i]);
+            } else {
+                y[i] = x[i];
+            }
+        }
+        memcpy(x, y, sizeof(double) * (size_t)n);
+    }
+    free(y);
+    return ORC_OK;
+}

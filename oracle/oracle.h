@@ -123,6 +123,9 @@ int orc_cg_ir(const orc_csr *A, const double *b, double *x, double tol, int max_
               double inner_tol, int inner_max, double *hist, int *outer_iters,
               int *inner_iters_total);
 
+/* Damped Jacobi smoothing of a guess x in place (sweeps, 0 < omega < 2). */
+int orc_jacobi_smooth(const orc_csr *A, const double *b, double *x, int sweeps, double omega);
+
 /* ---------------------------------------------------------- generators -- */
 
 uint64_t orc_splitmix64_next(uint64_t *state);

@@ -63,6 +63,7 @@ def lib():
                                        _dp, _ip]
         L.orc_cg_ir.argtypes = [ctypes.POINTER(_Csr), _dp, _dp, c_double, c_int, c_double, c_int, _dp,
                                 _ip, _ip]
+        L.orc_jacobi_smooth.argtypes = [ctypes.POINTER(_Csr), _dp, _dp, c_int, c_double]
         L.orc_splitmix64_next.argtypes = [ctypes.POINTER(c_u64)]
         L.orc_splitmix64_next.restype = c_u64
         L.orc_stream_state.argtypes = [c_u64, c_u64]
@@ -188,6 +189,14 @@ def cg_ir(A, b, x0=None, tol=1e-8, max_outer=50, inner_tol=1e-4, inner_max=10000
     st = lib().orc_cg_ir(ctypes.byref(c), _ptr(b, _dp), _ptr(x, _dp), tol, max_outer, inner_tol,
                          inner_max, _ptr(hist, _dp), ctypes.byref(oi), ctypes.byref(ii))
     return st, x, hist[: oi.value + 1], oi.value, ii.value
+
+
+def jacobi_smooth(A, b, x, sweeps, omega=0.8):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.array(x, dtype=np.float64, copy=True)
+    c = A._c()
+    lib().orc_jacobi_smooth(ctypes.byref(c), _ptr(b, _dp), _ptr(x, _dp), sweeps, omega)
+    return x
 
 
 # -------------------------------------------------------------- generators --

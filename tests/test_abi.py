@@ -124,3 +124,30 @@ def test_unet_create_rejects_bad_shapes(cfm, n, batch):
     with pytest.raises(cfm.CfmError) as e:
         cfm.UNet(n, batch)
     assert e.value.status == cfm.EINVAL
+
+
+def test_varcoef_coefficients_match_matrix(cfm):
+    """The per-cell k reproduces every matrix entry: off-diagonal
+    -2 k_i k_j / (k_i + k_j), boundary faces k_i, diagonal the sum of the
+    six faces in the order z-, y-, x-, x+, y+, z+."""
+    rp, col, val, f, dims = cfm.varcoef3d(11, 4, 3, 7)
+    k = cfm.varcoef3d_coef(11, 4, 3, 7)
+    assert k.shape == (dims[2], dims[1], dims[0])
+    nx, ny, nz = dims
+    kf = k.reshape(-1)
+    for i in range(kf.size):
+        ix, iy, iz = i % nx, (i // nx) % ny, i // (nx * ny)
+        nbs = [i - nx * ny if iz > 0 else -1, i - nx if iy > 0 else -1, i - 1 if ix > 0 else -1,
+               i + 1 if ix < nx - 1 else -1, i + nx if iy < ny - 1 else -1, i + nx * ny if iz < nz - 1 else -1]
+        cf = [(2.0 * kf[i] * kf[j]) / (kf[i] + kf[j]) if j >= 0 else kf[i] for j in nbs]
+        diag = 0.0
+        for c in cf:
+            diag = diag + c
+        row = dict(zip(col[rp[i]:rp[i + 1]], val[rp[i]:rp[i + 1]]))
+        assert row.pop(i) == diag
+        for j, c in zip(nbs, cf):
+            if j >= 0:
+                assert row.pop(j) == -c
+        assert not row
+    lk = np.log(cfm.varcoef3d_coef(11, 5, 32, 32))
+    assert abs(lk.mean()) < 0.01 and abs(lk.std() - 1.0) < 0.01

@@ -163,3 +163,108 @@ def test_batch_converges_true_residual(gpu):
         xi = x[srp[i]:srp[i + 1]]
         r = f_ - M @ xi
         assert np.linalg.norm(r) <= 2e-8 * np.linalg.norm(f_)
+
+
+@pytest.mark.parametrize("block", [256, 1024])
+def test_batch_x0_guard_rejects_bad_guesses(gpu, orc, block):
+    """Guard on: a guess with ||b - A x0|| > ||b|| restarts from zero and then
+    equals the oracle's run from zero bit for bit; an acceptable guess is
+    kept and equals the oracle's run from that guess."""
+    systems = _systems(gpu, 21, 8, 6, 18)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-9, max_iter=3000, x0_guard=1, block_threads=block)
+    rng = np.random.default_rng(0)
+    x0 = np.zeros(b.size)
+    bad = np.array([i % 3 == 0 for i in range(len(systems))])
+    for i, (rp_, col_, val_, f) in enumerate(systems):
+        r0, r1 = srp[i], srp[i + 1]
+        if bad[i]:  # noise only raises the residual: ||b - A x0||^2 ~ ||b||^2 + ||A x0||^2
+            x0[r0:r1] = 1e-2 * rng.standard_normal(r1 - r0)
+        else:  # half the solution: b - A x0 ~ b / 2
+            x0[r0:r1] = 0.5 * orc.cg(orc.Csr(rp_, col_, val_), f, tol=1e-12, max_iter=5000)[1]
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(x0)
+        st, iters, stat, hist, t = plan.solve(hist_cap=3001)
+        x = plan.x()
+        rej = plan.x0_rejected()
+    assert st == gpu.OK
+    assert np.array_equal(rej.astype(bool), bad)
+    for i, (rp_, col_, val_, f) in enumerate(systems):
+        r0, r1 = srp[i], srp[i + 1]
+        ost, ox, oh, oit = orc.cg(orc.Csr(rp_, col_, val_), f, x0=None if bad[i] else x0[r0:r1], tol=1e-9,
+                                  max_iter=3000, variant="pipelined")
+        assert iters[i] == oit, i
+        assert np.array_equal(x[r0:r1], ox), i
+        assert np.array_equal(hist[i, : oit + 1], oh), i
+
+
+def test_batch_x0_guard_off_keeps_bad_guess(gpu, orc):
+    systems = _systems(gpu, 22, 3, 6, 10)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-9, max_iter=3000, x0_guard=0)
+    x0 = 1e2 * np.random.default_rng(1).standard_normal(b.size)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(x0)
+        st, iters, stat, hist, t = plan.solve(hist_cap=3001)
+        x = plan.x()
+        assert not plan.x0_rejected().any()
+    _check_members(orc, systems, srp, x, iters, hist, 1e-9, 3000, x0=x0)
+
+
+def test_batch_warm_start_writes_network_output(gpu):
+    """x0 after warm_start == scale * net(features), for a system count that
+    is not a multiple of the network's batch (short last group)."""
+    n, nsys, scale = 8, 5, 0.37
+    systems, feats = [], []
+    rng = np.random.default_rng(3)
+    for sid in range(nsys):
+        rp, col, val, f, dims = gpu.varcoef3d(9, sid, n, n)
+        systems.append((rp, col, val, f))
+        feats.append(rng.standard_normal((2, n ** 3)).astype(np.float32))
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=0.0, max_iter=0)
+    params = gpu.unet_init_params(4)
+    params[-1] = 0.25  # head bias, so that no output is trivially zero
+    with gpu.UNet(n, 2, params=params) as net, gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.load_features(np.concatenate([f_.reshape(-1) for f_ in feats]))
+        tw = plan.warm_start(net, scale)
+        plan.solve()  # zero iterations: x = x0
+        x = plan.x()
+        expect = []
+        for g in range(0, nsys, 2):
+            grp = feats[g: g + 2] + [np.zeros((2, n ** 3), np.float32)] * (2 - len(feats[g: g + 2]))
+            out = net.forward(np.stack(grp))
+            expect += [scale * out[j].reshape(-1).astype(np.float64) for j in range(min(2, nsys - g))]
+    assert tw["iterations"] == 3
+    assert np.array_equal(x, np.concatenate(expect))
+
+
+def test_batch_warm_start_rejects_mismatched_sizes(gpu):
+    systems = _systems(gpu, 23, 2, 6, 10)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-8, max_iter=100)
+    with gpu.UNet(8, 1) as net, gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        with pytest.raises(gpu.CfmError) as e:
+            plan.warm_start(net, 1.0)  # no features loaded
+        assert e.value.status == gpu.EINVAL
+        plan.load_features(np.zeros(2 * b.size, np.float32))
+        with pytest.raises(gpu.CfmError) as e:
+            plan.warm_start(net, 1.0)  # systems are not 8^3 cubes
+        assert e.value.status == gpu.EINVAL
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_batch_smooth_x0_equals_oracle(gpu, orc, sweeps):
+    systems = _systems(gpu, 24, 5, 5, 20)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=0.0, max_iter=0)
+    x0 = np.random.default_rng(5).standard_normal(b.size)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(x0)
+        t = plan.smooth_x0(sweeps, 0.8)
+        plan.solve()  # zero iterations: x = x0
+        x = plan.x()
+    assert t["iterations"] == sweeps
+    for i, (rp_, col_, val_, f) in enumerate(systems):
+        r0, r1 = srp[i], srp[i + 1]
+        ref = orc.jacobi_smooth(orc.Csr(rp_, col_, val_), f, x0[r0:r1], sweeps, 0.8)
+        assert np.array_equal(x[r0:r1], ref), i

@@ -226,3 +226,25 @@ def test_varcoef_dims_are_log_uniform(orc):
     assert draws.min() >= 32 and draws.max() <= 96
     # log-uniform: the median is near sqrt(32*97) = 55.7
     assert 50 <= np.median(draws) <= 62
+
+
+def test_jacobi_smooth_pins(orc):
+    import scipy.sparse as sps
+    A = orc.poisson3d(6, 5, 4)
+    M = sps.csr_matrix((A.val, A.col, A.rowptr))
+    rng = np.random.default_rng(8)
+    xi = rng.integers(-5, 6, A.n).astype(np.float64)
+    b = M @ xi  # integer arithmetic: exact, so xi is a fixed point bit for bit
+    assert np.array_equal(orc.jacobi_smooth(A, b, xi, 3, 0.8), xi)
+    # one sweep equals the textbook damped Jacobi update
+    x0 = rng.standard_normal(A.n)
+    b = rng.standard_normal(A.n)
+    ref = x0 + 0.7 * (b - M @ x0) / M.diagonal()
+    assert np.allclose(orc.jacobi_smooth(A, b, x0, 1, 0.7), ref, rtol=1e-13, atol=1e-13)
+    # two sweeps compose
+    one = orc.jacobi_smooth(A, b, x0, 1, 0.7)
+    assert np.array_equal(orc.jacobi_smooth(A, b, one, 1, 0.7), orc.jacobi_smooth(A, b, x0, 2, 0.7))
+    # sweeps reduce the error of an SPD system (omega < 1)
+    xs = np.linalg.solve(M.toarray(), b)
+    e0 = np.linalg.norm(x0 - xs)
+    assert np.linalg.norm(orc.jacobi_smooth(A, b, x0, 20, 0.8) - xs) < 0.5 * e0

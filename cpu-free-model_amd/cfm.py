@@ -29,7 +29,7 @@ CG_PERSISTENT, CG_DISCRETE = 0, 1
 
 # Every symbol include/cfm.h declares.
 EXPORTS = (
-    "cfm_version", "cfm_status_str", "cfm_device_count", "cfm_device_info", "cfm_device_synchronize",
+    "cfm_version", "cfm_status_str", "cfm_last_error", "cfm_device_count", "cfm_device_info", "cfm_device_synchronize",
     "cfm_bandwidth_probe",
     "cfm_jacobi_plan_create", "cfm_jacobi_plan_load", "cfm_jacobi_plan_run", "cfm_jacobi_plan_download",
     "cfm_jacobi_plan_field", "cfm_jacobi_plan_destroy", "cfm_jacobi_run", "cfm_jacobi_init_field",
@@ -47,7 +47,8 @@ EXPORTS = (
 class CfmError(RuntimeError):
     def __init__(self, status, where=""):
         self.status = status
-        super().__init__(f"{where}: {status_str(status)} ({status})")
+        msg = last_error()
+        super().__init__(f"{where}: {status_str(status)} ({status})" + (f": {msg}" if msg else ""))
 
 
 class JacobiParams(ctypes.Structure):
@@ -77,13 +78,33 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 
 
+def _loaded_hip_runtimes():
+    try:
+        with open("/proc/self/maps") as f:
+            return {ln.split()[-1] for ln in f if "libamdhip64" in ln and "/" in ln}
+    except OSError:
+        return set()
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C cpu-free-model_amd)")
+    # libcfm binds to whichever libamdhip64.so.7 the process already holds (the
+    # soname is shared).  A different runtime loaded first -- PyTorch wheels
+    # bundle their own -- would run these kernels on a runtime they were not
+    # built for, so refuse instead of running on it.
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    want = os.path.realpath(os.path.join(rocm, "lib", "libamdhip64.so.7"))
+    other = {p for p in _loaded_hip_runtimes() if os.path.realpath(p) != want}
+    if other and os.path.exists(want):
+        raise ImportError(f"cfm: this process already loaded the HIP runtime {sorted(other)[0]} (for example by "
+                          f"importing torch first); libcfm needs {want}.  Import cfm before torch.")
     L = ctypes.CDLL(LIB_PATH)
     L.cfm_version.restype = ctypes.c_char_p
     L.cfm_status_str.restype = ctypes.c_char_p
     L.cfm_status_str.argtypes = [ctypes.c_int]
+    L.cfm_last_error.argtypes = []
+    L.cfm_last_error.restype = ctypes.c_char_p
     L.cfm_device_count.argtypes = [_ip]
     L.cfm_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _ip,
                                   ctypes.POINTER(ctypes.c_size_t)]
@@ -148,6 +169,11 @@ lib = _load()
 
 def status_str(s):
     return lib.cfm_status_str(int(s)).decode()
+
+
+def last_error():
+    m = lib.cfm_last_error()
+    return m.decode() if m else ""
 
 
 def version():

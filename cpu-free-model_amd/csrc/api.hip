@@ -3,12 +3,25 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
 #include "cfm_common.hpp"
 
 extern "C" const char *cfm_version(void) { return "cfm 0.1.0 (gfx950, HIP)"; }
+
+static thread_local char g_last_error[1024] = "";
+
+void cfm::set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *cfm_last_error(void) { return g_last_error; }
 
 extern "C" const char *cfm_status_str(int s) {
     switch (s) {

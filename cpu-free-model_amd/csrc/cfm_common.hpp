@@ -12,16 +12,22 @@ namespace cfm {
 
 constexpr int kWave = 64;  // CDNA wavefront width
 
-// Status of the last failing HIP call is printed (stderr) and mapped to a
-// cfm_status; nothing here aborts the process.
-#define CFM_HIP(call)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            std::fprintf(stderr, "cfm: HIP call \"%s\" failed at %s:%d: %s\n", #call,   \
-                         __FILE__, __LINE__, hipGetErrorString(e_));                   \
-            return (e_ == hipErrorOutOfMemory) ? CFM_ENOMEM : CFM_EDEVICE;             \
-        }                                                                              \
+// Records the message returned by cfm_last_error() (per host thread).
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+// Returns `code` after recording a message for cfm_last_error().
+#define CFM_FAIL(code, ...) (::cfm::set_error(__VA_ARGS__), (code))
+
+// A failing HIP call records its message and maps to a cfm_status; nothing
+// here aborts the process.
+#define CFM_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::cfm::set_error("HIP call \"%s\" failed at %s:%d: %s", #call, __FILE__, __LINE__,  \
+                           hipGetErrorString(e_));                                            \
+            return (e_ == hipErrorOutOfMemory) ? CFM_ENOMEM : CFM_EDEVICE;                    \
+        }                                                                                     \
     } while (0)
 
 // ---------------------------------------------------------------- barrier --

@@ -464,9 +464,10 @@ static cfm_status batch_configure(cfm_cg_batch_plan *pl, int sub, int grid) {
     BatchKernelFn k = batch_kernel(sub);
     if (!k) return CFM_EINVAL;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, 256 * sub, 0) != hipSuccess ||
-        per_cu < 1)
-        return CFM_EDEVICE;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, 256 * sub, 0);
+    if (e != hipSuccess || per_cu < 1)
+        return CFM_FAIL(CFM_EDEVICE, "batched CG: occupancy query for %d threads gave %d workgroups per CU (%s)",
+                        256 * sub, per_cu, hipGetErrorString(e));
     const int max_grid = std::min(per_cu * pl->dev.cus, kMaxGrid);
     pl->sub = sub;
     pl->grid = std::max(1, std::min(grid > 0 ? grid : max_grid, max_grid));
@@ -476,13 +477,15 @@ static cfm_status batch_configure(cfm_cg_batch_plan *pl, int sub, int grid) {
 extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_ptr, const int *rowptr,
                                                const int *col, const double *val, const cfm_cg_params *p,
                                                cfm_cg_batch_plan **out) {
-    if (!out) return CFM_EINVAL;
+    if (!out) return CFM_FAIL(CFM_EINVAL, "batched CG: null output handle");
     *out = nullptr;
-    if (nsys < 1 || nsys > kMaxSys || !sys_row_ptr || !rowptr || !col || !val || !p) return CFM_EINVAL;
-    if (p->max_iter < 0 || p->method != CFM_CG_PIPELINED || p->mode != CFM_CG_PERSISTENT) return CFM_EINVAL;
-    if (sys_row_ptr[0] != 0) return CFM_EINVAL;
+    if (nsys < 1 || nsys > kMaxSys || !sys_row_ptr || !rowptr || !col || !val || !p)
+        return CFM_FAIL(CFM_EINVAL, "batched CG: need 1..%d systems and non-null arrays (nsys %d)", kMaxSys, nsys);
+    if (p->max_iter < 0 || p->method != CFM_CG_PIPELINED || p->mode != CFM_CG_PERSISTENT)
+        return CFM_FAIL(CFM_EINVAL, "batched CG: Pipelined persistent method and max_iter >= 0 required");
+    if (sys_row_ptr[0] != 0) return CFM_FAIL(CFM_EINVAL, "batched CG: sys_row_ptr[0] must be 0");
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_EDEVICE;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_FAIL(CFM_EDEVICE, "no GPU");
     auto *pl = new (std::nothrow) cfm_cg_batch_plan();
     if (!pl) return CFM_ENOMEM;
     auto fail = [&](cfm_status s) {
@@ -499,11 +502,11 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
     long long chunks = 0;
     for (int sy = 0; sy < nsys; ++sy) {
         const long long nr = sys_row_ptr[sy + 1] - sys_row_ptr[sy];
-        if (nr < 1 || nr > 0x3fffffff) return fail(CFM_EINVAL);
+        if (nr < 1 || nr > 0x3fffffff) return fail(CFM_FAIL(CFM_EINVAL, "batched CG: system %d has %lld rows", sy, nr));
         pl->h_rows[sy] = (int)nr;
         pl->h_chunk0[sy] = (int)chunks;
         chunks += (nr + kChunk - 1) / kChunk;
-        if (chunks * kChunk > 0x7fffffffLL) return fail(CFM_EINVAL);
+        if (chunks * kChunk > 0x7fffffffLL) return fail(CFM_FAIL(CFM_EINVAL, "batched CG: padded rows exceed 2^31"));
     }
     pl->h_chunk0[nsys] = (int)chunks;
     pl->total_chunks = (int)chunks;
@@ -526,7 +529,7 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
             }
             sp[sl] = (int)tot;
             tot += 64LL * wdt;
-            if (tot > 0x7fffffffLL) return fail(CFM_EINVAL);
+            if (tot > 0x7fffffffLL) return fail(CFM_FAIL(CFM_EINVAL, "batched CG: SELL storage exceeds 2^31 entries"));
         }
     }
     sp[nslices] = (int)tot;
@@ -542,7 +545,9 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
             const int lane = (int)(prow % 64);
             for (int j = rowptr[g0 + lr]; j < rowptr[g0 + lr + 1]; ++j) {
                 const long long cg = col[j];
-                if (cg < g0 || cg >= g1) return fail(CFM_EINVAL);  // not block diagonal
+                if (cg < g0 || cg >= g1)  // not block diagonal
+                    return fail(CFM_FAIL(CFM_EINVAL, "batched CG: row %lld of system %d has column %lld outside it",
+                                         g0 + lr, sy, cg));
                 const size_t e = (size_t)sp[sl] + (size_t)(j - rowptr[g0 + lr]) * 64 + lane;
                 sc[e] = (int)(prow0 + (cg - g0));
                 sv[e] = val[j];
@@ -565,7 +570,7 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
               dalloc((void **)&pl->sync, sizeof(GridSync));
     double **vecs[] = {&pl->b, &pl->x0, &pl->x, &pl->r, &pl->z, &pl->s, &pl->pv, &pl->w[0], &pl->w[1]};
     for (double **v : vecs) ok = ok && dalloc((void **)v, sizeof(double) * nv);
-    if (!ok) return fail(CFM_ENOMEM);
+    if (!ok) return fail(CFM_FAIL(CFM_ENOMEM, "batched CG: device allocation failed"));
     if (hipMemcpy(pl->sys_chunk0, pl->h_chunk0.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(pl->sys_rows, pl->h_rows.data(), sizeof(int) * nsys, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(pl->slice_ptr, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -578,7 +583,9 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
         hipEventCreate(&pl->ev1) != hipSuccess)
         return fail(CFM_EDEVICE);
     const int bt = p->block_threads > 0 ? p->block_threads : 1024;
-    if (bt % 256 || batch_configure(pl, bt / 256, p->grid_blocks) != CFM_OK) return fail(CFM_EINVAL);
+    if (bt % 256 || bt > 1024) return fail(CFM_FAIL(CFM_EINVAL, "batched CG: block_threads %d", bt));
+    const cfm_status cs = batch_configure(pl, bt / 256, p->grid_blocks);
+    if (cs != CFM_OK) return fail(cs);
     int rate_khz = 0;
     const unsigned long long us = p->spin_limit_us > 0 ? p->spin_limit_us : 10000000ull;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, pl->dev.device) != hipSuccess ||

@@ -95,6 +95,9 @@ typedef struct cfm_cg_plan cfm_cg_plan;
 /* ------------------------------------------------------------ library --- */
 const char *cfm_version(void);
 const char *cfm_status_str(int status);
+/* Message of the last failure recorded on the calling host thread ("" if
+ * none): the failed check or HIP call. */
+const char *cfm_last_error(void);
 cfm_status cfm_device_count(int *count);
 /* name, compute units, HBM bytes of device `dev` */
 cfm_status cfm_device_info(int dev, char *name, int name_cap, int *compute_units,

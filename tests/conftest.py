@@ -10,6 +10,10 @@ for p in (PKG, ORACLE, REPO):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# Load libcfm (and with it the system HIP runtime) before any test module
+# imports torch, whose wheel bundles a HIP runtime of its own.
+import cfm  # noqa: E402,F401
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path through the C ABI)")

@@ -151,3 +151,20 @@ def test_varcoef_coefficients_match_matrix(cfm):
         assert not row
     lk = np.log(cfm.varcoef3d_coef(11, 5, 32, 32))
     assert abs(lk.mean()) < 0.01 and abs(lk.std() - 1.0) < 0.01
+
+
+def test_refuses_foreign_hip_runtime():
+    """Importing cfm after a library that brought its own HIP runtime (the
+    torch wheel) fails loudly instead of running kernels on that runtime."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import torch\n"
+            "try:\n    import cfm\nexcept ImportError as e:\n    print('refused', 'torch' in str(e))\n"
+            "else:\n    print('loaded')\n") % os.path.join(REPO, "cpu-free-model_amd")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    torch_dir = subprocess.run([sys.executable, "-c", "import torch, os; print(os.path.dirname(torch.__file__))"],
+                               capture_output=True, text=True, timeout=300).stdout.strip()
+    if not os.path.exists(os.path.join(torch_dir, "lib", "libamdhip64.so")):
+        pytest.skip("this torch build does not bundle a HIP runtime")
+    assert out.stdout.strip() == "refused True", out.stdout + out.stderr

@@ -8,13 +8,21 @@ hand-worked ties and torch's bfloat16 conversion.
 """
 import numpy as np
 import pytest
-import torch
-import torch.nn.functional as F
 
 import unet_oracle as uo
 
 
-def test_bf16_rounding_known_answers():
+@pytest.fixture(scope="module")
+def tt():
+    """torch and torch.nn.functional, imported only when these CPU tests run
+    (so that a GPU test process never loads the HIP runtime torch bundles)."""
+    import torch
+    import torch.nn.functional as F
+    return torch, F
+
+
+def test_bf16_rounding_known_answers(tt):
+    torch, F = tt
     one = 1.0
     vals = np.array([one, one + 2.0 ** -8, one + 3 * 2.0 ** -8, one + 2.0 ** -7, -2.5, 0.0, 3.0e38])
     exp = np.array([one, one, one + 2.0 ** -6, one + 2.0 ** -7, -2.5, 0.0,
@@ -22,7 +30,8 @@ def test_bf16_rounding_known_answers():
     assert np.array_equal(uo.bf16(vals), exp)
 
 
-def test_bf16_rounding_matches_torch():
+def test_bf16_rounding_matches_torch(tt):
+    torch, F = tt
     x = np.random.default_rng(0).standard_normal(100000) * 10.0 ** np.random.default_rng(1).uniform(-8, 8, 100000)
     t = torch.from_numpy(x.astype(np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
     assert np.array_equal(uo.bf16(x), t)
@@ -48,7 +57,8 @@ def _brute_conv(X, W, b):
     return Y
 
 
-def test_conv_brute_force_and_torch():
+def test_conv_brute_force_and_torch(tt):
+    torch, F = tt
     rng = np.random.default_rng(2)
     X = rng.standard_normal((2, 4, 4, 4, 3))
     W = rng.standard_normal((5, 3, 27))
@@ -74,7 +84,8 @@ def test_conv_adjoint_identity():
     assert np.sum(uo.conv3(X, V) * Y) == pytest.approx(np.sum(V * uo.conv3_wgrad(X, Y)), rel=1e-12)
 
 
-def test_pool_and_upsample_match_torch():
+def test_pool_and_upsample_match_torch(tt):
+    torch, F = tt
     rng = np.random.default_rng(4)
     X = rng.standard_normal((2, 8, 8, 8, 3))
     Xt = torch.from_numpy(np.moveaxis(X, -1, 1).copy())
@@ -88,7 +99,8 @@ def test_pool_and_upsample_match_torch():
     assert np.sum(uo.pool(X) * Z) == pytest.approx(np.sum(X * uo.pool_bwd(Z)), rel=1e-12)
 
 
-def _torch_net(params, inp, target, n):
+def _torch_net(tt, params, inp, target, n):
+    torch, F = tt
     convs, (hw, hb) = uo.split_params(params)
     P = [torch.tensor(p, dtype=torch.float64, requires_grad=True)
          for w, b in convs for p in (w.reshape(w.shape[0], w.shape[1], 3, 3, 3), b)]
@@ -124,11 +136,12 @@ def _case(seed, n, batch):
     return params, inp, target
 
 
-def test_network_matches_torch_autograd():
+def test_network_matches_torch_autograd(tt):
+    torch, F = tt
     n = 8
     params, inp, target = _case(5, n, 2)
     loss, g, pred = uo.loss_and_grad(params, inp, target, n)
-    tl, tg, tp = _torch_net(params, inp, target, n)
+    tl, tg, tp = _torch_net(tt, params, inp, target, n)
     assert loss == pytest.approx(tl, rel=1e-12)
     assert np.allclose(pred, tp, rtol=1e-11, atol=1e-13)
     assert np.linalg.norm(g - tg) <= 1e-10 * np.linalg.norm(tg)

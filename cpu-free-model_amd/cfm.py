@@ -54,7 +54,7 @@ class CfmError(RuntimeError):
 class JacobiParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "nx", "ny", "nz", "niter", "norm_every", "prec", "bc", "variant", "tile_rows", "grid_blocks",
-        "chunks", "spin_limit_us", "block_threads")]
+        "chunks", "spin_limit_us", "block_threads", "time_block")]
 
 
 class Timing(ctypes.Structure):
@@ -220,9 +220,10 @@ def init_field(nx, ny, nz=1, kind=INIT_SIN, prec=F64):
 
 
 def jacobi_params(nx, ny, nz=1, niter=1000, norm_every=0, prec=F64, bc=BC_DIRICHLET,
-                  variant=PERSISTENT, tile_rows=0, grid_blocks=0, chunks=0, spin_limit_us=0, block_threads=0):
+                  variant=PERSISTENT, tile_rows=0, grid_blocks=0, chunks=0, spin_limit_us=0, block_threads=0,
+                  time_block=0):
     return JacobiParams(nx, ny, nz, niter, norm_every, prec, bc, variant, tile_rows, grid_blocks, chunks,
-                        spin_limit_us, block_threads)
+                        spin_limit_us, block_threads, time_block)
 
 
 class JacobiPlan:

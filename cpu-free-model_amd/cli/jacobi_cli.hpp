@@ -4,8 +4,11 @@
 //
 // One GPU: the multi-GPU versions collapse onto the two single-GPU schemes.
 // "Baseline *" run one launch per sweep (the reference's single_gpu loop),
-// "Design 1/2" and "PERKS" run the single persistent cooperative kernel, and
-// the "(No computation)" versions run the persistent kernel's barriers only.
+// "Design 1/2" run the single persistent cooperative kernel one sweep per
+// pass, "PERKS" the persistent kernel that keeps the intermediate sweep on
+// chip (two sweeps per HBM pass where the shape allows it, -tblock 1|2 to
+// force), and the "(No computation)" versions run the persistent kernel's
+// barriers only.
 //
 // -compare runs the launch-per-sweep scheme first (the reference's
 // single_gpu, Stencil/jacobi2D/src/common.cu:154-273) and compares every
@@ -55,6 +58,7 @@ inline int run_jacobi(int argc, char **argv, int dims) {
     p.grid_blocks = get_argval<int>(b, e, "-grid", 0);
     p.chunks = get_argval<int>(b, e, "-chunks", 0);
     p.block_threads = get_argval<int>(b, e, "-block", 0);
+    p.time_block = get_argval<int>(b, e, "-tblock", selection == 5 ? 0 : 1);
     const bool json = get_arg(b, e, "-json");
     p.prec = prec == "f32" ? CFM_F32 : CFM_F64;
     p.bc = bc == "dirichlet" ? CFM_BC_DIRICHLET : CFM_BC_PERIODIC;

@@ -341,9 +341,11 @@ __device__ __forceinline__ void sweep3d_wgtile(const T *__restrict__ src, T *__r
     const bool ld_el = wx == 0 && lane == 0 && x0 - 1 >= 0;             // column x0-1
     const bool ld_er = wx == WX - 1 && lane == 63 && x0 + Tl::TW < nx;  // column x0+TW
 
-    T prev[RY][VX], cur[RY][VX], nxt[RY][VX];
-    T hcur[VX] = {}, hnxt[VX] = {};  // this wave's halo row (top or bottom wave only)
-    T ecur[RY] = {}, enxt[RY] = {};  // this lane's edge column (x-edge lanes only)
+    // planes z-1, z, z+1 in use and z+2 in flight: the loads issued at step z
+    // are consumed at step z+1, so a whole step covers their latency
+    T prev[RY][VX], cur[RY][VX], nxt[RY][VX], nx2[RY][VX];
+    T hcur[VX] = {}, hnxt[VX] = {}, hnx2[VX] = {};  // this wave's halo row (top or bottom wave only)
+    T ecur[RY] = {}, enxt[RY] = {}, enx2[RY] = {};  // this lane's edge column (x-edge lanes only)
 
     auto load_plane = [&](int z, T (&rows)[RY][VX], T (&h)[VX], T (&e)[RY]) {
         const T *pz = src + (size_t)z * plane;
@@ -371,9 +373,10 @@ __device__ __forceinline__ void sweep3d_wgtile(const T *__restrict__ src, T *__r
         T hd[VX], ed[RY];
         load_plane(zb - 1, prev, hd, ed);
         load_plane(zb, cur, hcur, ecur);
+        load_plane(zb + 1, nxt, hnxt, enxt);  // zb+1 <= ze <= nz-1
     }
     for (int z = zb; z < ze; ++z) {
-        load_plane(z + 1, nxt, hnxt, enxt);  // z+1 <= nz-1
+        if (z + 2 <= nz - 1) load_plane(z + 2, nx2, hnx2, enx2);  // uniform over the workgroup
         typename Tl::Plane &P = lds[z & 1];
         // publish this wave's boundary rows / columns of plane z
 #pragma unroll
@@ -472,11 +475,18 @@ __device__ __forceinline__ void sweep3d_wgtile(const T *__restrict__ src, T *__r
             for (int e = 0; e < VX; ++e) {
                 prev[r][e] = cur[r][e];
                 cur[r][e] = nxt[r][e];
+                nxt[r][e] = nx2[r][e];
             }
 #pragma unroll
-        for (int e = 0; e < VX; ++e) hcur[e] = hnxt[e];
+        for (int e = 0; e < VX; ++e) {
+            hcur[e] = hnxt[e];
+            hnxt[e] = hnx2[e];
+        }
 #pragma unroll
-        for (int r = 0; r < RY; ++r) ecur[r] = enxt[r];
+        for (int r = 0; r < RY; ++r) {
+            ecur[r] = enxt[r];
+            enxt[r] = enx2[r];
+        }
     }
     __syncthreads();  // the LDS planes are reused by the next unit
 }
@@ -511,6 +521,298 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A)
             }
         }
         if (k + 1 < A.niter) {
+            if (!grid_barrier(A.sync, epoch)) return;
+        }
+    }
+}
+
+// ------------------------------------------------- 3D, two sweeps per pass --
+// Temporal blocking for the Dirichlet case when one tile spans the x extent
+// (nx <= TW): a pass reads the field at sweep t once and writes sweep t+2,
+// so HBM sees one read and one write per two updates.  The intermediate
+// sweep t+1 ("level 1") lives only in registers and LDS.
+//
+// Workgroup = WX x WY wavefronts; wave (wx, wy) owns columns
+// [wx*SEG, wx*SEG + SEG) of row y = ty*WY + wy and streams planes p of a
+// z-chunk.  At step p it computes level 1 on plane p from level-0 planes
+// p-1, p, p+1, publishes its level-1 row and x-edge values to LDS, and after
+// the step's one __syncthreads computes level 2 on plane p-1 from level-1
+// planes p-2, p-1, p and the y-neighbour rows of the waves above and below.
+// The tile's top and bottom waves also compute level 1 on the row just
+// outside the tile (redundantly with the neighbouring tile), and every chunk
+// recomputes level 1 on the plane just outside each end.  Every level-1 value
+// comes from the same expression as a single sweep and boundary points keep
+// their value, so the result is bit-identical to two single sweeps.
+//
+// Level-0 rows of a wave: "column" rows are needed on planes p-1, p, p+1
+// (its own row, and the tile-halo row of the top/bottom wave) and are
+// prefetched CLA planes ahead -- they are the HBM stream; "side" rows are
+// needed on plane p only (the y-neighbours of the level-1 rows) and are
+// fetched one plane ahead (the neighbouring wave or tile has streamed them
+// into L2).  x-edge values of the column rows come from the neighbouring
+// waves through LDS.  The pass is bound by instruction issue rather than by
+// HBM, so the per-row work is kept free of per-lane selects and masks.  LDS is triple-buffered by plane (written
+// at step p, read at step p+1, rewritten at step p+3).
+template <typename T, int WX, int WY>
+struct T2Tile {
+    static constexpr int VX = 16 / sizeof(T);
+    static constexpr int SEG = 64 * VX;
+    static constexpr int TW = SEG * WX;
+    static constexpr int BLOCK = 64 * WX * WY;
+    struct Buf {
+        T row[WY][TW];         // level-1 own row of each wave (plane p)
+        T l1e[WY][WX][2];      // its first (lane 0) and last (lane 63) element
+        T l0e[WY][WX][2][2];   // level-0 column rows of plane p+1: [slot][first, last]
+    };
+};
+
+// FULL: nx == TW, so every lane's VX columns lie inside the rows (unmasked
+// vector loads and stores; the two boundary columns are rewritten with their
+// own, unchanged value).  Wave roles (top / middle / bottom of the tile) are
+// wave-uniform branches, so no per-lane selects are spent on them.
+template <typename T, int WX, int WY, bool ALIGNED, bool FULL, int CLA = 1>
+__device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
+                                           int ty, int zc, bool single, bool want0, bool want1, double &n0,
+                                           double &n1, typename T2Tile<T, WX, WY>::Buf *lds) {
+    using Tl = T2Tile<T, WX, WY>;
+    using V = typename Vec16<T>::type;
+    constexpr int VX = Tl::VX;
+    static_assert(WY >= 2, "the top and bottom wave must differ");
+    static_assert(!FULL || ALIGNED, "full-width rows are 16-byte aligned");
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wx = wave % WX, wy = wave / WX;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const size_t plane = (size_t)nx * ny;
+    const int x = wx * Tl::SEG + lane * VX;
+    const int y = ty * WY + wy;
+    const int zb = 1 + zc * A.chunk_len;
+    const int ze = min(zb + A.chunk_len, nz - 1);
+    if (zb >= ze) return;  // uniform over the workgroup
+    const T c = stencil_coeff3d<T>();
+    const bool top = wy == 0, bot = wy == WY - 1;
+    const bool lane_w = lane == 0, lane_e = lane == 63;
+    // column rows: [0] own row, [1] tile-halo row (top: y-1, bottom: y+1)
+    const int yc1 = top ? y - 1 : y + 1;
+    const bool has_h = top || bot;
+    // side rows (plane p only): [0] below the lowest level-1 row, [1] above the highest
+    const int ys0 = top ? y - 2 : y - 1;
+    const int ys1 = bot ? y + 2 : y + 1;
+    bool xin[VX];  // interior columns of this lane
+#pragma unroll
+    for (int k = 0; k < VX; ++k) xin[k] = x + k >= 1 && x + k <= nx - 2;
+
+    constexpr int NC = 3 + CLA;  // column rows, planes p-1 .. p+1+CLA (the last CLA in flight)
+    T col[NC][2][VX];
+    T sid[2][2][VX];  // side rows, planes p and p+1 (p+1 in flight)
+    T l1o[3][VX];     // level-1 own row, planes p-2, p-1, p
+    T l1h[2][VX];     // level-1 tile-halo row (top/bottom waves), planes p-1, p
+
+    auto ld = [&](int z, int yy, bool use, T (&v)[VX]) {
+        if (use && z >= 0 && z < nz && yy >= 0 && yy < ny) {  // wave-uniform
+            const T *pr = src + (size_t)z * plane + (size_t)yy * nx;
+            if (FULL) {
+                const V t = *reinterpret_cast<const V *>(pr + x);
+#pragma unroll
+                for (int k = 0; k < VX; ++k) v[k] = t[k];
+            } else {
+                load_row<T, VX, ALIGNED>(pr, x, nx, true, v);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) v[k] = T(0);
+        }
+    };
+    auto ld_col = [&](int z, T (&v)[2][VX]) {
+        ld(z, y, true, v[0]);
+        ld(z, yc1, has_h, v[1]);
+    };
+    auto ld_side = [&](int z, T (&v)[2][VX]) {
+        ld(z, ys0, true, v[0]);
+        ld(z, ys1, true, v[1]);
+    };
+    auto put_l0e = [&](typename Tl::Buf &B, const T (&v)[2][VX]) {
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            if (lane_w) B.l0e[wy][wx][sl][0] = v[sl][0];
+            if (lane_e) B.l0e[wy][wx][sl][1] = v[sl][VX - 1];
+        }
+    };
+    // one level-1 row: out = stencil where interior, else the centre value
+    auto l1row = [&](const typename Tl::Buf &B, int sl, int yy, bool pin, const T (&cr)[VX], const T (&N)[VX],
+                     const T (&S)[VX], const T (&U)[VX], const T (&D)[VX], T (&out)[VX], bool norm) {
+        T west = __shfl_up(cr[VX - 1], 1, 64);
+        T east = __shfl_down(cr[0], 1, 64);
+        if (lane_w) west = wx > 0 ? B.l0e[wy][wx > 0 ? wx - 1 : 0][sl][1] : T(0);
+        if (lane_e) east = wx < WX - 1 ? B.l0e[wy][wx < WX - 1 ? wx + 1 : 0][sl][0] : T(0);
+        if (pin && yy >= 1 && yy <= ny - 2) {  // wave-uniform
+#pragma unroll
+            for (int k = 0; k < VX; ++k) {
+                const T E = (k < VX - 1) ? cr[k + 1] : east;
+                const T W = (k > 0) ? cr[k - 1] : west;
+                const T v = c * (E + W + N[k] + S[k] + U[k] + D[k]);
+                out[k] = xin[k] ? v : cr[k];
+                if (norm && xin[k]) {
+                    const double d = (double)v - (double)cr[k];
+                    n0 += d * d;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) out[k] = cr[k];
+        }
+    };
+    auto store_row = [&](int z, const T (&out)[VX], const bool (&inter)[VX]) {
+        T *drow = dst + (size_t)z * plane + (size_t)y * nx;
+        if (FULL) {
+            V t;
+#pragma unroll
+            for (int k = 0; k < VX; ++k) t[k] = out[k];
+            store16(reinterpret_cast<V *>(drow + x), t);
+        } else if (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2) {
+            V t;
+#pragma unroll
+            for (int k = 0; k < VX; ++k) t[k] = out[k];
+            store16(reinterpret_cast<V *>(drow + x), t);
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k)
+                if (inter[k]) drow[x + k] = out[k];
+        }
+    };
+
+    const int p0 = single ? zb : zb - 1;
+    const int p1 = single ? ze - 1 : ze;  // inclusive
+#pragma unroll
+    for (int i = 0; i < NC - 1; ++i) ld_col(p0 - 1 + i, col[i]);
+    ld_side(p0, sid[0]);
+    put_l0e(lds[p0 % 3], col[1]);
+    __syncthreads();
+    for (int p = p0; p <= p1; ++p) {
+        ld_col(p + 1 + CLA, col[NC - 1]);  // consumed at step p+CLA
+        ld_side(p + 1, sid[1]);  // consumed at step p+1
+        typename Tl::Buf &Bp = lds[p % 3];
+        const bool pin = p >= 1 && p <= nz - 2;  // interior plane
+        const bool nrm = want0 && p >= zb && p < ze;
+        // ---- level 1 on plane p: own row, and the tile-halo row of the top/bottom wave
+        if (top) {
+            l1row(Bp, 0, y, pin, col[1][0], sid[0][1], col[1][1], col[2][0], col[0][0], l1o[2], nrm);
+            l1row(Bp, 1, y - 1, pin, col[1][1], col[1][0], sid[0][0], col[2][1], col[0][1], l1h[1], false);
+        } else if (bot) {
+            l1row(Bp, 0, y, pin, col[1][0], col[1][1], sid[0][0], col[2][0], col[0][0], l1o[2], nrm);
+            l1row(Bp, 1, y + 1, pin, col[1][1], sid[0][1], col[1][0], col[2][1], col[0][1], l1h[1], false);
+        } else {
+            l1row(Bp, 0, y, pin, col[1][0], sid[0][1], sid[0][0], col[2][0], col[0][0], l1o[2], nrm);
+        }
+        typename Tl::Buf &Bn = lds[(p + 1) % 3];
+        if (single) {
+            // one sweep only: store the own level-1 row of plane p
+            if (y >= 1 && y <= ny - 2) store_row(p, l1o[2], xin);
+            put_l0e(Bn, col[2]);
+            __syncthreads();
+        } else {
+            // ---- publish the own level-1 row of plane p and the level-0 edges of plane p+1
+            {
+                V t;
+#pragma unroll
+                for (int k = 0; k < VX; ++k) t[k] = l1o[2][k];
+                *reinterpret_cast<V *>(&Bp.row[wy][x]) = t;
+            }
+            if (lane_w) Bp.l1e[wy][wx][0] = l1o[2][0];
+            if (lane_e) Bp.l1e[wy][wx][1] = l1o[2][VX - 1];
+            put_l0e(Bn, col[2]);
+            __syncthreads();
+            // ---- level 2 on plane p-1
+            const int zo = p - 1;
+            if (zo >= zb && zo >= 1 && zo <= nz - 2 && y >= 1 && y <= ny - 2) {  // wave-uniform
+                const typename Tl::Buf &R = lds[(p + 2) % 3];  // written at step p-1
+                T N[VX], S[VX];
+                if (top) {
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) S[k] = l1h[0][k];
+                } else {
+                    const V t = *reinterpret_cast<const V *>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x]);
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) S[k] = t[k];
+                }
+                if (bot) {
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) N[k] = l1h[0][k];
+                } else {
+                    const V t = *reinterpret_cast<const V *>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x]);
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) N[k] = t[k];
+                }
+                const T *cr = l1o[1];
+                T west = __shfl_up(cr[VX - 1], 1, 64);
+                T east = __shfl_down(cr[0], 1, 64);
+                if (lane_w) west = wx > 0 ? R.l1e[wy][wx > 0 ? wx - 1 : 0][1] : T(0);
+                if (lane_e) east = wx < WX - 1 ? R.l1e[wy][wx < WX - 1 ? wx + 1 : 0][0] : T(0);
+                T out[VX];
+#pragma unroll
+                for (int k = 0; k < VX; ++k) {
+                    const T E = (k < VX - 1) ? cr[k + 1] : east;
+                    const T Wv = (k > 0) ? cr[k - 1] : west;
+                    const T v = c * (E + Wv + N[k] + S[k] + l1o[2][k] + l1o[0][k]);
+                    out[k] = xin[k] ? v : cr[k];
+                    if (want1 && xin[k]) {
+                        const double d = (double)v - (double)cr[k];
+                        n1 += d * d;
+                    }
+                }
+                store_row(zo, out, xin);
+            }
+        }
+        // ---- slide the windows
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+#pragma unroll
+                for (int i = 0; i < NC - 1; ++i) col[i][sl][k] = col[i + 1][sl][k];
+                sid[0][sl][k] = sid[1][sl][k];
+            }
+            l1o[0][k] = l1o[1][k];
+            l1o[1][k] = l1o[2][k];
+            l1h[0][k] = l1h[1][k];
+        }
+    }
+    __syncthreads();  // the LDS buffers are reused by the next unit
+}
+
+template <typename T, int WX, int WY, bool ALIGNED, bool FULL>
+__global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
+    using Tl = T2Tile<T, WX, WY>;
+    __shared__ typename Tl::Buf bufs[3];
+    __shared__ double red[Tl::BLOCK / 64];
+    const int g = gridDim.x, b = blockIdx.x;
+    const int lb = (g & 7) == 0 ? (b & 7) * (g >> 3) + (b >> 3) : b;  // XCD-contiguous order
+    const int nunits = A.nty * A.chunks;
+    unsigned int epoch = 0;
+    int pass = 0;
+    for (int k = 0; k < A.niter; ++pass) {
+        const int steps = min(2, A.niter - k);
+        const int it = A.it0 + k;
+        const bool want0 = A.norm_every > 0 && (it % A.norm_every) == 0;
+        const bool want1 = steps == 2 && A.norm_every > 0 && ((it + 1) % A.norm_every) == 0;
+        const T *src = static_cast<const T *>((pass & 1) ? A.buf1 : A.buf0);
+        T *dst = static_cast<T *>((pass & 1) ? A.buf0 : A.buf1);
+        double n0 = 0.0, n1 = 0.0;
+        for (int u = lb; u < nunits; u += g) {
+            const int zc = u / A.nty;
+            const int ty = u - zc * A.nty;
+            sweep3d_t2<T, WX, WY, ALIGNED, FULL>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1, bufs);
+        }
+        if (want0) {
+            const double s = block_sum_waves<Tl::BLOCK / 64>(n0, red);
+            if (threadIdx.x == 0) A.norm_part[(size_t)(it / A.norm_every) * gridDim.x + blockIdx.x] = s;
+        }
+        if (want1) {
+            const double s = block_sum_waves<Tl::BLOCK / 64>(n1, red);
+            if (threadIdx.x == 0) A.norm_part[(size_t)((it + 1) / A.norm_every) * gridDim.x + blockIdx.x] = s;
+        }
+        k += steps;
+        if (k < A.niter) {
             if (!grid_barrier(A.sync, epoch)) return;
         }
     }
@@ -693,6 +995,17 @@ static KernelFn pick_kernel_t(bool is3d, int rows, int block) {
 #undef CFM_J2
 }
 
+// Two sweeps per pass: FP64 4 x 4 wavefronts (512 columns x 4 rows), FP32
+// 2 x 8 (512 columns x 8 rows); both tiles are 512 wide.
+constexpr int kT2Width = 512;
+template <typename T>
+static KernelFn pick_t2_kernel(bool aligned, bool full) {
+    constexpr int WX = sizeof(T) == 8 ? 4 : 2, WY = sizeof(T) == 8 ? 4 : 8;
+    if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, true, true>;
+    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, true, false>
+                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, false, false>;
+}
+
 template <typename T>
 static KernelFn pick_kernel(bool is3d, int rows, bool aligned, int block) {
     return aligned ? pick_kernel_t<T, true>(is3d, rows, block) : pick_kernel_t<T, false>(is3d, rows, block);
@@ -727,6 +1040,7 @@ struct cfm_jacobi_plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     KernelFn kernel = nullptr;
+    bool t2 = false;  // two sweeps per pass (jacobi3d_t2_kernel)
     int grid = 0;
     int chunks = 0, chunk_len = 0, ntx = 0, nty = 0, rows = 0, block = 256;
     unsigned long long spin_ticks = 0;
@@ -740,6 +1054,7 @@ static cfm_status validate(const cfm_jacobi_params *p) {
     if (p->bc != CFM_BC_PERIODIC && p->bc != CFM_BC_DIRICHLET) return CFM_EINVAL;
     if (p->variant < CFM_JACOBI_LAUNCH_PER_ITER || p->variant > CFM_JACOBI_NO_COMPUTE)
         return CFM_EINVAL;
+    if (p->time_block < 0 || p->time_block > 2) return CFM_EINVAL;
     // 32-bit plane/row arithmetic inside the kernels
     const double elems = (double)p->nx * p->ny * p->nz;
     if (elems > 9.0e15 || (double)p->nx * p->ny > 2.0e9) return CFM_EINVAL;
@@ -768,14 +1083,30 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     const int vx = 16 / (int)pl->elem_size;
     const bool aligned = ((size_t)p->nx * pl->elem_size) % 16 == 0;
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
-    pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
-    pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows);
-    if (!is3d && pl->block != 256 && pl->block != default_block(false, pl->rows)) {
+    // two sweeps per pass: persistent 3D Dirichlet with one tile across x
+    const bool t2_ok = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
+                       p->nx <= kT2Width && p->tile_rows <= 0 && (p->block_threads <= 0 || p->block_threads == 1024);
+    if (p->time_block == 2 && !t2_ok) {
         delete pl;
-        return CFM_EINVAL;
+        return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent 3D Dirichlet variant, "
+                                     "nx <= %d and the default tile shape", kT2Width);
     }
-    pl->kernel = f64 ? pick_kernel<double>(is3d, pl->rows, aligned, pl->block)
-                     : pick_kernel<float>(is3d, pl->rows, aligned, pl->block);
+    pl->t2 = t2_ok && p->time_block != 1;
+    if (pl->t2) {
+        pl->rows = 1;
+        pl->block = 1024;
+        const bool full = p->nx == kT2Width;
+        pl->kernel = f64 ? pick_t2_kernel<double>(aligned, full) : pick_t2_kernel<float>(aligned, full);
+    } else {
+        pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
+        pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows);
+        if (!is3d && pl->block != 256 && pl->block != default_block(false, pl->rows)) {
+            delete pl;
+            return CFM_EINVAL;
+        }
+        pl->kernel = f64 ? pick_kernel<double>(is3d, pl->rows, aligned, pl->block)
+                         : pick_kernel<float>(is3d, pl->rows, aligned, pl->block);
+    }
     if (!pl->kernel) {
         delete pl;
         return CFM_EINVAL;
@@ -793,8 +1124,12 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     // work units: one per wavefront (per-wave kernels) or per workgroup (3D tile kernel)
     const bool wg_tile = is3d && pl->block != 256;
     const int nworkers = wg_tile ? pl->grid : pl->grid * (pl->block / 64);
-    const int tile_w = wg_tile ? 64 * vx * 4 : 64 * vx;
-    const int tile_h = wg_tile ? pl->rows * (pl->block / 256) : pl->rows;
+    int tile_w = wg_tile ? 64 * vx * 4 : 64 * vx;
+    int tile_h = wg_tile ? pl->rows * (pl->block / 256) : pl->rows;
+    if (pl->t2) {
+        tile_w = kT2Width;
+        tile_h = f64 ? 4 : 8;
+    }
     pl->ntx = (p->nx + tile_w - 1) / tile_w;
     const int interior = is3d ? p->nz - 2 : p->ny - 2;
     pl->nty = is3d ? (p->ny + tile_h - 1) / tile_h : 1;
@@ -802,7 +1137,8 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     int chunks = p->chunks;
     if (chunks <= 0) {
         // 3D tiles of 2 rows: 4 z-chunks per workgroup balance best; else one
-        const int per_worker = (wg_tile && pl->rows <= 2) ? 4 : 1;
+        // (two sweeps per pass: one unit per workgroup, long chunks)
+        const int per_worker = (wg_tile && !pl->t2 && pl->rows <= 2) ? 4 : 1;
         chunks = std::max(1, per_worker * nworkers / tiles);
         const int min_len = is3d ? 16 : 64;
         chunks = std::min(chunks, std::max(1, interior / min_len));
@@ -907,14 +1243,14 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
         produced = pl->norm_slots;
     }
     if (norm_count) *norm_count = produced;
-    if (p.variant != CFM_JACOBI_NO_COMPUTE) pl->cur ^= (p.niter & 1);
+    if (p.variant != CFM_JACOBI_NO_COMPUTE) pl->cur ^= pl->t2 ? (((p.niter + 1) / 2) & 1) : (p.niter & 1);
     if (t) {
         t->kernel_ms = ms;
         t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
         t->grid_blocks = pl->grid;
         t->block_threads = pl->block;
         t->iterations = p.niter;
-        t->outer_iterations = 0;
+        t->outer_iterations = p.variant == CFM_JACOBI_NO_COMPUTE ? 0 : pl->t2 ? (p.niter + 1) / 2 : p.niter;
     }
     return CFM_OK;
 }

@@ -78,6 +78,8 @@ typedef struct {
     int chunks;       /* chunks along the streamed axis; 0 = auto */
     int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
     int block_threads;/* 0 = one workgroup per CU (size set by the tile shape), or 256 */
+    int time_block;   /* sweeps per HBM pass of the persistent 3D kernel: 0 = auto (2 for
+                         Dirichlet with nx <= 512 and the default tile shape, else 1), 1, 2 */
 } cfm_jacobi_params;
 
 typedef struct {
@@ -86,7 +88,7 @@ typedef struct {
     int grid_blocks;    /* workgroups launched */
     int block_threads;  /* threads per workgroup */
     int iterations;     /* sweeps (Jacobi) or CG iterations (summed inner for IR) */
-    int outer_iterations;
+    int outer_iterations; /* Jacobi: passes over the field in HBM; IR: refinements */
 } cfm_timing;
 
 typedef struct cfm_jacobi_plan cfm_jacobi_plan;

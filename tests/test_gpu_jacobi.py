@@ -142,3 +142,40 @@ def test_full_size_2d_properties(gpu):
     assert np.array_equal(out, out[:, ::-1])
     assert out[1:-1, 1:-1].min() >= init.min() and out[1:-1, 1:-1].max() <= init.max()
     assert np.all(np.diff(norms) < 0)
+
+
+T2_SHAPES = [(3, 5, 6), (9, 13, 512), (17, 23, 511), (40, 37, 100), (12, 70, 257)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", T2_SHAPES)
+@pytest.mark.parametrize("niter", [1, 2, 3, 8])
+def test_two_sweeps_per_pass_bitexact(gpu, orc, dtype, shape, niter):
+    """Temporal blocking (level 1 kept on chip) equals single sweeps bit for bit."""
+    init = _rand(shape, dtype, 11 + niter)
+    _check(gpu, orc, init, niter, 1, time_block=2)
+
+
+@pytest.mark.parametrize("norm_every", [1, 2, 3])
+@pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (5, 0), (0, 1), (3, 7)])
+def test_two_sweeps_per_pass_norms_and_units(gpu, orc, norm_every, chunks, grid):
+    init = _rand((26, 45, 130), np.float64, 5)
+    _check(gpu, orc, init, 9, 1, norm_every=norm_every, time_block=2, chunks=chunks, grid_blocks=grid)
+
+
+def test_two_sweeps_per_pass_matches_one_sweep_per_pass(gpu):
+    init = _rand((20, 31, 512), np.float64, 8)
+    a, na, ta = gpu.jacobi(init, 10, bc=1, norm_every=5, time_block=1)
+    b, nb, tb = gpu.jacobi(init, 10, bc=1, norm_every=5, time_block=2)
+    assert np.array_equal(a, b)
+    assert np.allclose(na, nb, rtol=NORM_RTOL, atol=0.0)
+
+
+@pytest.mark.parametrize("kw", [dict(bc=0), dict(bc=1, nx=600), dict(bc=1, variant=0), dict(bc=1, tile_rows=2)])
+def test_two_sweeps_per_pass_rejects_unsupported(gpu, kw):
+    nx = kw.pop("nx", 64)
+    bc = kw.pop("bc")
+    p = gpu.jacobi_params(nx, 16, 8, niter=2, bc=bc, time_block=2, **kw)
+    with pytest.raises(gpu.CfmError) as e:
+        gpu.JacobiPlan(p)
+    assert e.value.status == gpu.ENOTSUP

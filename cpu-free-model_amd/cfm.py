@@ -70,7 +70,8 @@ class CgParams(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("mode", ctypes.c_int), ("tol", ctypes.c_double),
                 ("max_iter", ctypes.c_int), ("restart_quirk", ctypes.c_int), ("max_outer", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("grid_blocks", ctypes.c_int),
-                ("spin_limit_us", ctypes.c_int), ("block_threads", ctypes.c_int), ("x0_guard", ctypes.c_int)]
+                ("spin_limit_us", ctypes.c_int), ("block_threads", ctypes.c_int), ("x0_guard", ctypes.c_int),
+                ("matrix_format", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -318,9 +319,10 @@ def uniform(seed, stream_id, count):
 
 
 def cg_params(method=CG_STANDARD, mode=CG_PERSISTENT, tol=1e-8, max_iter=10000, restart_quirk=1,
-              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0, block_threads=0, x0_guard=0):
+              max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0, block_threads=0, x0_guard=0,
+              matrix_format=0):
     return CgParams(method, mode, tol, max_iter, restart_quirk, max_outer, inner_tol, grid_blocks,
-                    spin_limit_us, block_threads, x0_guard)
+                    spin_limit_us, block_threads, x0_guard, matrix_format)
 
 
 class CgPlan:

@@ -10,16 +10,39 @@ constexpr int kChunk = 1024;
 constexpr int kThreads = 256;
 constexpr int kSlicesPerChunk = kChunk / 64;
 
+// Matrix view of the CG kernels.  SELL-64 (ndiag == 0): slices of 64 rows,
+// entry j of the 64 rows of slice s at slice_ptr[s] + 64 j + lane, column -1
+// for padding.  DIA (ndiag > 0, banded matrices with sorted rows): diagonal d
+// holds a(row, row + doff[d]) at val[d * npad + row], 0 where the CSR has no
+// entry; no column indices are stored.  Both give each row's sum in the CSR's
+// stored column order (a stored 0 adds an exact 0 to a non-negative-zero sum).
 struct Sell {
     const int *slice_ptr;  // [nslices + 1], entry offsets (multiples of 64)
     const int *col;        // [slice_ptr[nslices]], -1 = padding
     const void *val;       // double or float
+    const int *doff;       // DIA: [ndiag] ascending column offsets
+    int ndiag;             // 0 = SELL-64
+    int n;                 // DIA: rows (columns outside [0, n) are skipped)
+    int npad;              // DIA: stride between diagonals
 };
 
 
 // ------------------------------------------------------------ SpMV row ----
 template <typename VT, typename XT>
 __device__ __forceinline__ XT sell_row(const Sell &S, const XT *__restrict__ x, int slice, int lane) {
+    if (S.ndiag > 0) {  // uniform over the grid
+        const int row = slice * 64 + lane;
+        const VT *vp = static_cast<const VT *>(S.val) + row;
+        XT acc = XT(0);
+        // one diagonal at a time: an 8-wide unrolled form pushed the persistent
+        // pipelined kernel into scratch memory
+        for (int d = 0; d < S.ndiag; ++d) {
+            const int c = row + S.doff[d];
+            const VT v = vp[(size_t)d * S.npad];
+            if (c >= 0 && c < S.n) acc = fma((XT)v, x[c], acc);
+        }
+        return acc;
+    }
     const int beg = S.slice_ptr[slice];
     const int width = (S.slice_ptr[slice + 1] - beg) >> 6;
     const int *cp = S.col + beg + lane;

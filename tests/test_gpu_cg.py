@@ -178,3 +178,31 @@ def test_full_size_128_ir(gpu):
     A = sp.csr_matrix((val, col, rp))
     r = b - A @ x
     assert np.sqrt(r @ r) <= 1e-8 * np.sqrt(b @ b)
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+@pytest.mark.parametrize("method", ["standard", "pipelined", "ir"])
+def test_dia_and_sell_storage_equal_oracle(gpu, orc, fmt, method):
+    """The index-free DIA storage (banded CSR) and SELL-64 give the same bits."""
+    rp, col, val, b = _poisson(gpu, (20, 13, 9), seed=4)
+    m = {"standard": gpu.CG_STANDARD, "pipelined": gpu.CG_PIPELINED, "ir": gpu.CG_IR_F32}[method]
+    g = gpu.cg(rp, col, val, b, method=m, tol=1e-10, max_iter=3000, matrix_format=fmt)
+    _same(g, _oracle(orc, rp, col, val, b, method, 1e-10, 3000))
+
+
+def test_dia_on_varcoef_and_one_based(gpu, orc):
+    rp, col, val, f, dims = gpu.varcoef3d(5, 2, 7, 15)
+    g = gpu.cg(rp + 1, col + 1, val, f, index_base=1, method=gpu.CG_PIPELINED, tol=1e-9, max_iter=3000,
+               matrix_format=2)
+    _same(g, _oracle(orc, rp + 1, col + 1, val, f, "pipelined", 1e-9, 3000, base=1))
+
+
+def test_dia_rejects_unbanded(gpu):
+    n = 3000
+    rng = np.random.default_rng(2)
+    M = sp.random(n, n, density=0.004, random_state=3, format="csr")
+    M = (M + M.T + sp.eye(n) * 50).tocsr()
+    M.sort_indices()
+    with pytest.raises(gpu.CfmError) as e:
+        gpu.cg(M.indptr, M.indices, M.data, rng.standard_normal(n), matrix_format=2)
+    assert e.value.status == gpu.ENOTSUP

@@ -77,9 +77,16 @@ def cpu_baseline(n, target_s):
 
 
 def cg_figures(cfm):
+    """Time-to-1e-8 of the 128^3 Poisson system per method, and the FP32-in-FP64
+    refinement's solution error against the FP64 Standard CG solution (which
+    equals the oracle's bit for bit) with both true residuals."""
+    import numpy as np
+    import scipy.sparse as sps
     out = {}
     rp, col, val = cfm.poisson3d(128, 128, 128)
     b = cfm.uniform(0, 0, rp.size - 1)
+    M = sps.csr_matrix((val, col, rp))
+    xs = {}
     for name, method, mode in (("standard", cfm.CG_STANDARD, cfm.CG_PERSISTENT),
                                ("pipelined", cfm.CG_PIPELINED, cfm.CG_PERSISTENT),
                                ("ir_f32", cfm.CG_IR_F32, cfm.CG_PERSISTENT),
@@ -99,7 +106,37 @@ def cg_figures(cfm):
             out[name] = {"ms_to_1e-8": round(t["kernel_ms"], 3), "iterations": it,
                          "outer": t["outer_iterations"], "status": cfm.status_str(st),
                          "us_per_iter": round(1000 * t["kernel_ms"] / max(it, 1), 2)}
+            if name in ("standard", "ir_f32"):
+                xs[name] = plan.x()
+    x64, x32 = xs["standard"], xs["ir_f32"]
+    nb = float(np.linalg.norm(b))
+    out["ir_f32"]["rel_err_vs_f64"] = float(f"{np.linalg.norm(x32 - x64) / np.linalg.norm(x64):.3e}")
+    out["ir_f32"]["true_rel_residual"] = float(f"{np.linalg.norm(b - M @ x32) / nb:.3e}")
+    out["standard"]["true_rel_residual"] = float(f"{np.linalg.norm(b - M @ x64) / nb:.3e}")
     return out
+
+
+def jacobi_f32_figures(cfm, n, niter):
+    """FP32 sweep (FP64 norms) against the FP64 sweep, whose field equals the
+    oracle's bit for bit: rate and the field's error after niter sweeps."""
+    import numpy as np
+    res = {}
+    fields = {}
+    for prec, name in ((cfm.F64, "f64"), (cfm.F32, "f32")):
+        init = cfm.init_field(n, n, n, cfm.INIT_SIN, prec)
+        p = cfm.jacobi_params(n, n, n, niter=niter, prec=prec, bc=cfm.BC_DIRICHLET, variant=cfm.PERSISTENT)
+        with cfm.JacobiPlan(p) as plan:
+            plan.load(init)
+            del init
+            plan.run()
+            plan.load(cfm.init_field(n, n, n, cfm.INIT_SIN, prec))
+            _, t = plan.run()
+            fields[name] = plan.download()
+        res[name] = {"glups": round((n - 2) ** 3 * niter / t.kernel_ms / 1e6, 1), "ms": round(t.kernel_ms, 2)}
+    d = fields["f32"].astype(np.float64) - fields["f64"]
+    res["f32"]["max_abs_err_vs_f64"] = float(f"{np.abs(d).max():.3e}")
+    res["f32"]["rel_l2_err_vs_f64"] = float(f"{np.linalg.norm(d) / np.linalg.norm(fields['f64']):.3e}")
+    return res
 
 
 def cg_batch_figures(cfm, nsys, lmin=32, lmax=96, seed=2024):
@@ -224,6 +261,8 @@ def main():
     }
     if not args.no_cg:
         line["cg_128"] = cg_figures(cfm)
+    if not args.no_cg:
+        line["jacobi_precision"] = jacobi_f32_figures(cfm, n, niter)
     if not args.no_batch:
         line["cg_batch"] = cg_batch_figures(cfm, args.batch_systems)
     if not args.no_warmstart:

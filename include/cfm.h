@@ -150,7 +150,7 @@ typedef struct {
     int max_iter;        /* CG iterations (IR: inner iterations per refinement) */
     int restart_quirk;   /* pipelined: 1 = reference's `k > 1` restart (default), 0 = `k > 0` */
     int max_outer;       /* IR: refinement steps (0 = 50) */
-    double inner_tol;    /* IR: inner relative tolerance (0 = 1e-4) */
+    double inner_tol;    /* IR: inner relative tolerance (0 = 1e-5), never below half the outer tol */
     int grid_blocks;     /* 0 = auto */
     int spin_limit_us;   /* device barrier watchdog; 0 = default (10 s) */
     int block_threads;   /* 256, 512, 768 or 1024 (256-row sub-blocks per workgroup); 0 = auto */

@@ -269,7 +269,8 @@ int orc_cg_ir(const orc_csr *A, const double *b, double *x, double tol, int max_
             d[i] = 0.0f;
         }
         double g = orc_dot_f32(rf, rf, n, ORC_TREE_GPU);
-        const double thr_in = inner_tol * inner_tol * g;
+        /* no refinement is solved beyond half the outer target */
+        const double thr_in = fmax(inner_tol * inner_tol * g, 0.25 * thr);
         int j = 0, brk = 0;
         while (j < inner_max) {
             spmv_f32(n, A->rowptr, A->col, val32, A->base, p, s);

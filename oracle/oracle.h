@@ -116,7 +116,9 @@ int orc_cg_pipelined(const orc_csr *A, const double *b, double *x, double tol, i
 
 /* FP64 iterative refinement around an inner CG whose SpMV and vectors are
  * FP32 (dot products accumulate in FP64).  inner_tol is relative to the
- * outer residual; the outer loop stops when ||b - A x||^2 <= tol^2 ||b||^2.
+ * outer residual, floored at half the outer target (inner <r,r> <=
+ * max(inner_tol^2 <r0,r0>, tol^2 <b,b> / 4)); the outer loop stops when
+ * ||b - A x||^2 <= tol^2 ||b||^2.
  * hist receives the outer <r,r> after each refinement (cap max_outer+1);
  * inner_iters_total receives the summed inner iteration count. */
 int orc_cg_ir(const orc_csr *A, const double *b, double *x, double tol, int max_outer,

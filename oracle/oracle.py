@@ -178,7 +178,7 @@ def cg(A, b, x0=None, tol=1e-8, max_iter=10000, variant="standard", tree=TREE_GP
     return st, x, hist[: it.value + 1], it.value
 
 
-def cg_ir(A, b, x0=None, tol=1e-8, max_outer=50, inner_tol=1e-4, inner_max=10000):
+def cg_ir(A, b, x0=None, tol=1e-8, max_outer=50, inner_tol=1e-5, inner_max=10000):
     """FP32 inner CG inside an FP64 refinement loop.
     Returns (status, x, outer_hist, outer_iters, inner_total)."""
     b = np.ascontiguousarray(b, dtype=np.float64)

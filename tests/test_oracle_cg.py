@@ -248,3 +248,17 @@ def test_jacobi_smooth_pins(orc):
     xs = np.linalg.solve(M.toarray(), b)
     e0 = np.linalg.norm(x0 - xs)
     assert np.linalg.norm(orc.jacobi_smooth(A, b, x0, 20, 0.8) - xs) < 0.5 * e0
+
+
+def test_ir_last_refinement_stops_at_half_target(orc):
+    """The inner tolerance is floored at half the outer target: with a loose
+    outer tolerance one refinement suffices and lands between tol/2-ish and tol."""
+    A = orc.poisson3d(16, 16, 16)
+    b = orc.uniform(1, 0, A.n)
+    tol = 1e-3
+    st, x, hist, outer, inner = orc.cg_ir(A, b, tol=tol, inner_tol=1e-6)
+    assert st == orc.OK and outer == 1
+    rel = np.sqrt(hist[1] / hist[0])
+    assert rel <= tol
+    # without the floor the same inner tolerance would solve to ~1e-6
+    assert rel > 1e-5

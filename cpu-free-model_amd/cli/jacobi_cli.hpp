@@ -77,6 +77,7 @@ inline int run_jacobi(int argc, char **argv, int dims) {
         cfm_jacobi_params q = p;
         q.variant = CFM_JACOBI_LAUNCH_PER_ITER;
         q.norm_every = 0;
+        q.time_block = 1;  // the reference scheme: one sweep per launch
         if (dims == 2)
             std::printf("Single GPU jacobi relaxation (non-persistent kernel): %d iterations on %d x %d mesh "
                         "with norm check every %d iterations\n",

@@ -268,3 +268,29 @@ def test_batch_smooth_x0_equals_oracle(gpu, orc, sweeps):
         r0, r1 = srp[i], srp[i + 1]
         ref = orc.jacobi_smooth(orc.Csr(rp_, col_, val_), f, x0[r0:r1], sweeps, 0.8)
         assert np.array_equal(x[r0:r1], ref), i
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_batch_sell_and_dia_equal_oracle(gpu, orc, fmt):
+    """Both storages of the batch (SELL-64 and per-system DIA) give the oracle's bits."""
+    systems = _systems(gpu, 31, 10, 4, 17)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-10, max_iter=2000, matrix_format=fmt)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=2001)
+        x = plan.x()
+    assert st == gpu.OK
+    _check_members(orc, systems, srp, x, iters, hist, 1e-10, 2000)
+
+
+def test_batch_dia_rejects_unbanded(gpu):
+    n = 2000
+    M = sp.random(n, n, density=0.004, random_state=7, format="csr")
+    M = (M + M.T + sp.eye(n) * 50).tocsr()
+    M.sort_indices()
+    srp = np.array([0, n], dtype=np.int64)
+    p = gpu.cg_params(method=gpu.CG_PIPELINED, matrix_format=2)
+    with pytest.raises(gpu.CfmError) as e:
+        gpu.CgBatchPlan(srp, M.indptr, M.indices, M.data, p)
+    assert e.value.status == gpu.ENOTSUP

@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunks", type=int, default=0)
     ap.add_argument("--time-block", type=int, default=0, help="sweeps per HBM pass: 0 auto, 1, 2")
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -201,7 +202,7 @@ def main():
     init = cfm.init_field(n, n, n, cfm.INIT_SIN, cfm.F64)
     p = cfm.jacobi_params(n, n, n, niter=niter, prec=cfm.F64, bc=cfm.BC_DIRICHLET, variant=cfm.PERSISTENT,
                           tile_rows=args.tile_rows, grid_blocks=args.grid, chunks=args.chunks,
-                          time_block=args.time_block)
+                          time_block=args.time_block, block_threads=args.block)
     plan = cfm.JacobiPlan(p)
     plan.load(init)
     del init

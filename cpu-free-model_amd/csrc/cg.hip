@@ -744,11 +744,15 @@ extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, cons
     std::vector<int> sp, sc, doffs;
     std::vector<double> sv;
     const int npad = (n + kChunk - 1) / kChunk * kChunk;
-    // auto: DIA for Pipelined CG, whose fused iteration reads the matrix once per
-    // iteration alongside six vectors (measured 70 vs 80 us per iteration at 128^3);
-    // SELL-64 for Standard and the FP32 refinement, where it measured faster
+    // auto: DIA for Pipelined CG on large systems (measured 70 vs 80 us per
+    // iteration at 128^3 = 2.1M rows, where the index-free matrix and the vectors
+    // fit the MALL); SELL-64 for small systems (64^3: 27.8 vs 22.2 us per
+    // iteration, latency-bound, where the 8-wide SELL loop wins), for Standard CG
+    // and for the FP32 refinement
+    constexpr int kDiaMinRows = 1 << 20;
     bool dia = false;
-    if (p->matrix_format == 2 || (p->matrix_format == 0 && p->method == CFM_CG_PIPELINED)) {
+    if (p->matrix_format == 2 ||
+        (p->matrix_format == 0 && p->method == CFM_CG_PIPELINED && n >= kDiaMinRows)) {
         dia = build_dia(n, npad, rp, cp, vp, index_base, doffs, sv);
         if (!dia && p->matrix_format == 2)
             return CFM_FAIL(CFM_ENOTSUP, "cg: DIA storage needs a banded matrix (<= %d diagonals, sorted rows)",

@@ -74,6 +74,30 @@ struct Vec16<float> {
     using type = v4f32;
 };
 
+// VX consecutive elements as 16-byte vectors (p 16-byte aligned).
+template <typename T, int VX>
+__device__ __forceinline__ void ld_vec(const T *p, T (&v)[VX]) {
+    constexpr int E = 16 / sizeof(T);
+    static_assert(VX % E == 0, "whole 16-byte vectors");
+#pragma unroll
+    for (int q = 0; q < VX / E; ++q) {
+        const typename Vec16<T>::type t = *reinterpret_cast<const typename Vec16<T>::type *>(p + q * E);
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[q * E + e] = t[e];
+    }
+}
+template <typename T, int VX>
+__device__ __forceinline__ void st_vec(T *p, const T (&v)[VX]) {
+    constexpr int E = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < VX / E; ++q) {
+        typename Vec16<T>::type t;
+#pragma unroll
+        for (int e = 0; e < E; ++e) t[e] = v[q * E + e];
+        *reinterpret_cast<typename Vec16<T>::type *>(p + q * E) = t;
+    }
+}
+
 // Interior stores of the new field: it is not read again before the next
 // sweep, by which time it has left the caches, so it may bypass them.
 #ifndef CFM_JACOBI_NT_STORE
@@ -99,10 +123,7 @@ __device__ __forceinline__ void load_row(const T *__restrict__ p, int x, int nx,
                                          T (&v)[VX]) {
     if (ALIGNED) {
         if (row_ok && x + VX <= nx) {
-            const typename Vec16<T>::type t = *reinterpret_cast<const typename Vec16<T>::type *>(p + x);
-            static_assert(sizeof(t) == VX * sizeof(T), "vector width");
-#pragma unroll
-            for (int e = 0; e < VX; ++e) v[e] = t[e];
+            ld_vec<T, VX>(p + x, v);
             return;
         }
     }
@@ -553,9 +574,9 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A)
 // waves through LDS.  The pass is bound by instruction issue rather than by
 // HBM, so the per-row work is kept free of per-lane selects and masks.  LDS is triple-buffered by plane (written
 // at step p, read at step p+1, rewritten at step p+3).
-template <typename T, int WX, int WY>
+template <typename T, int WX, int WY, int VXT = 16 / sizeof(T)>
 struct T2Tile {
-    static constexpr int VX = 16 / sizeof(T);
+    static constexpr int VX = VXT;  // elements per lane and row (whole 16-byte vectors)
     static constexpr int SEG = 64 * VX;
     static constexpr int TW = SEG * WX;
     static constexpr int BLOCK = 64 * WX * WY;
@@ -570,12 +591,11 @@ struct T2Tile {
 // vector loads and stores; the two boundary columns are rewritten with their
 // own, unchanged value).  Wave roles (top / middle / bottom of the tile) are
 // wave-uniform branches, so no per-lane selects are spent on them.
-template <typename T, int WX, int WY, bool ALIGNED, bool FULL, int CLA = 1>
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, int CLA = 1>
 __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
                                            int ty, int zc, bool single, bool want0, bool want1, double &n0,
-                                           double &n1, typename T2Tile<T, WX, WY>::Buf *lds) {
-    using Tl = T2Tile<T, WX, WY>;
-    using V = typename Vec16<T>::type;
+                                           double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
+    using Tl = T2Tile<T, WX, WY, VXT>;
     constexpr int VX = Tl::VX;
     static_assert(WY >= 2, "the top and bottom wave must differ");
     static_assert(!FULL || ALIGNED, "full-width rows are 16-byte aligned");
@@ -612,9 +632,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
         if (use && z >= 0 && z < nz && yy >= 0 && yy < ny) {  // wave-uniform
             const T *pr = src + (size_t)z * plane + (size_t)yy * nx;
             if (FULL) {
-                const V t = *reinterpret_cast<const V *>(pr + x);
-#pragma unroll
-                for (int k = 0; k < VX; ++k) v[k] = t[k];
+                ld_vec<T, VX>(pr + x, v);
             } else {
                 load_row<T, VX, ALIGNED>(pr, x, nx, true, v);
             }
@@ -664,16 +682,15 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     };
     auto store_row = [&](int z, const T (&out)[VX], const bool (&inter)[VX]) {
         T *drow = dst + (size_t)z * plane + (size_t)y * nx;
-        if (FULL) {
-            V t;
+        constexpr int E = 16 / sizeof(T);
+        if (FULL || (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2)) {
 #pragma unroll
-            for (int k = 0; k < VX; ++k) t[k] = out[k];
-            store16(reinterpret_cast<V *>(drow + x), t);
-        } else if (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2) {
-            V t;
+            for (int q = 0; q < VX / E; ++q) {
+                typename Vec16<T>::type t;
 #pragma unroll
-            for (int k = 0; k < VX; ++k) t[k] = out[k];
-            store16(reinterpret_cast<V *>(drow + x), t);
+                for (int e = 0; e < E; ++e) t[e] = out[q * E + e];
+                store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E), t);
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < VX; ++k)
@@ -712,12 +729,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
             __syncthreads();
         } else {
             // ---- publish the own level-1 row of plane p and the level-0 edges of plane p+1
-            {
-                V t;
-#pragma unroll
-                for (int k = 0; k < VX; ++k) t[k] = l1o[2][k];
-                *reinterpret_cast<V *>(&Bp.row[wy][x]) = t;
-            }
+            st_vec<T, VX>(&Bp.row[wy][x], l1o[2]);
             if (lane_w) Bp.l1e[wy][wx][0] = l1o[2][0];
             if (lane_e) Bp.l1e[wy][wx][1] = l1o[2][VX - 1];
             put_l0e(Bn, col[2]);
@@ -731,17 +743,13 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
 #pragma unroll
                     for (int k = 0; k < VX; ++k) S[k] = l1h[0][k];
                 } else {
-                    const V t = *reinterpret_cast<const V *>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x]);
-#pragma unroll
-                    for (int k = 0; k < VX; ++k) S[k] = t[k];
+                    ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x], S);
                 }
                 if (bot) {
 #pragma unroll
                     for (int k = 0; k < VX; ++k) N[k] = l1h[0][k];
                 } else {
-                    const V t = *reinterpret_cast<const V *>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x]);
-#pragma unroll
-                    for (int k = 0; k < VX; ++k) N[k] = t[k];
+                    ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x], N);
                 }
                 const T *cr = l1o[1];
                 T west = __shfl_up(cr[VX - 1], 1, 64);
@@ -780,9 +788,9 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     __syncthreads();  // the LDS buffers are reused by the next unit
 }
 
-template <typename T, int WX, int WY, bool ALIGNED, bool FULL>
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
-    using Tl = T2Tile<T, WX, WY>;
+    using Tl = T2Tile<T, WX, WY, VXT>;
     __shared__ typename Tl::Buf bufs[3];
     __shared__ double red[Tl::BLOCK / 64];
     const int g = gridDim.x, b = blockIdx.x;
@@ -801,7 +809,7 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
         for (int u = lb; u < nunits; u += g) {
             const int zc = u / A.nty;
             const int ty = u - zc * A.nty;
-            sweep3d_t2<T, WX, WY, ALIGNED, FULL>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1, bufs);
+            sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1, bufs);
         }
         if (want0) {
             const double s = block_sum_waves<Tl::BLOCK / 64>(n0, red);
@@ -998,12 +1006,24 @@ static KernelFn pick_kernel_t(bool is3d, int rows, int block) {
 // Two sweeps per pass: FP64 4 x 4 wavefronts (512 columns x 4 rows), FP32
 // 2 x 8 (512 columns x 8 rows); both tiles are 512 wide.
 constexpr int kT2Width = 512;
+// Two sweeps per pass.  FP64: 4 x 4 wavefronts of 2 columns per lane (1024
+// threads) or, with 512 threads, 2 x 4 wavefronts of 4 columns per lane
+// (twice the registers per thread, half the per-wave overhead per point).
+// FP32: 2 x 8 wavefronts of 4 columns per lane.  All tiles are 512 wide.
+template <typename T, int WX, int WY, int VXT>
+static KernelFn pick_t2_shape(bool aligned, bool full) {
+    if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true>;
+    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, false>
+                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, false, false>;
+}
 template <typename T>
-static KernelFn pick_t2_kernel(bool aligned, bool full) {
-    constexpr int WX = sizeof(T) == 8 ? 4 : 2, WY = sizeof(T) == 8 ? 4 : 8;
-    if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, true, true>;
-    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, true, false>
-                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, false, false>;
+static KernelFn pick_t2_kernel(bool aligned, bool full, int block) {
+    if constexpr (sizeof(T) == 8) {
+        if (block == 512) return pick_t2_shape<T, 2, 4, 4>(aligned, full);
+        return pick_t2_shape<T, 4, 4, 2>(aligned, full);
+    } else {
+        return pick_t2_shape<T, 2, 8, 4>(aligned, full);
+    }
 }
 
 template <typename T>
@@ -1085,7 +1105,8 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
     // two sweeps per pass: persistent 3D Dirichlet with one tile across x
     const bool t2_ok = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
-                       p->nx <= kT2Width && p->tile_rows <= 0 && (p->block_threads <= 0 || p->block_threads == 1024);
+                       p->nx <= kT2Width && p->tile_rows <= 0 &&
+                       (p->block_threads <= 0 || p->block_threads == 1024 || (f64 && p->block_threads == 512));
     if (p->time_block == 2 && !t2_ok) {
         delete pl;
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent 3D Dirichlet variant, "
@@ -1094,9 +1115,11 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     pl->t2 = t2_ok && p->time_block != 1;
     if (pl->t2) {
         pl->rows = 1;
-        pl->block = 1024;
+        // FP64: 4 columns per lane in 512-thread workgroups measured 465 vs 440 GLUP/s at 512^3
+        pl->block = p->block_threads > 0 ? p->block_threads : (f64 ? 512 : 1024);
         const bool full = p->nx == kT2Width;
-        pl->kernel = f64 ? pick_t2_kernel<double>(aligned, full) : pick_t2_kernel<float>(aligned, full);
+        pl->kernel = f64 ? pick_t2_kernel<double>(aligned, full, pl->block)
+                         : pick_t2_kernel<float>(aligned, full, pl->block);
     } else {
         pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
         pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows);

@@ -156,9 +156,9 @@ typedef struct {
     int block_threads;   /* 256, 512, 768 or 1024 (256-row sub-blocks per workgroup); 0 = auto */
     int x0_guard;        /* batched plans: nonzero restarts a system from x0 = 0 when
                             ||b - A x0|| > ||b|| (the guard for learned initial guesses) */
-    int matrix_format;   /* 0 = auto (single-system Pipelined: DIA when the CSR is banded with
-                            <= 16 diagonals and sorted rows; otherwise SELL-64), 1 = SELL-64,
-                            2 = DIA (every system of a batch; ENOTSUP if not banded) */
+    int matrix_format;   /* 0 = auto (single-system Pipelined with >= 2^20 rows: DIA when the CSR
+                            is banded with <= 16 diagonals and sorted rows; otherwise SELL-64),
+                            1 = SELL-64, 2 = DIA (every system of a batch; ENOTSUP if not banded) */
 } cfm_cg_params;
 
 /* Replaces SingleStreamStandard::init / SingleStreamPipelined::init and

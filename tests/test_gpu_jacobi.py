@@ -147,20 +147,23 @@ def test_full_size_2d_properties(gpu):
 T2_SHAPES = [(3, 5, 6), (9, 13, 512), (17, 23, 511), (40, 37, 100), (12, 70, 257)]
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dtype,block", [(np.float64, 0), (np.float64, 512), (np.float32, 0)])
 @pytest.mark.parametrize("shape", T2_SHAPES)
 @pytest.mark.parametrize("niter", [1, 2, 3, 8])
-def test_two_sweeps_per_pass_bitexact(gpu, orc, dtype, shape, niter):
-    """Temporal blocking (level 1 kept on chip) equals single sweeps bit for bit."""
+def test_two_sweeps_per_pass_bitexact(gpu, orc, dtype, block, shape, niter):
+    """Temporal blocking (level 1 kept on chip) equals single sweeps bit for bit,
+    for both FP64 wavefront shapes (2 columns per lane x 16 waves, 4 x 8)."""
     init = _rand(shape, dtype, 11 + niter)
-    _check(gpu, orc, init, niter, 1, time_block=2)
+    _check(gpu, orc, init, niter, 1, time_block=2, block_threads=block)
 
 
 @pytest.mark.parametrize("norm_every", [1, 2, 3])
 @pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (5, 0), (0, 1), (3, 7)])
-def test_two_sweeps_per_pass_norms_and_units(gpu, orc, norm_every, chunks, grid):
+@pytest.mark.parametrize("block", [0, 512])
+def test_two_sweeps_per_pass_norms_and_units(gpu, orc, norm_every, chunks, grid, block):
     init = _rand((26, 45, 130), np.float64, 5)
-    _check(gpu, orc, init, 9, 1, norm_every=norm_every, time_block=2, chunks=chunks, grid_blocks=grid)
+    _check(gpu, orc, init, 9, 1, norm_every=norm_every, time_block=2, chunks=chunks, grid_blocks=grid,
+           block_threads=block)
 
 
 def test_two_sweeps_per_pass_matches_one_sweep_per_pass(gpu):

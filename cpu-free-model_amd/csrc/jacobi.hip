@@ -958,8 +958,208 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_kernel(JacobiArgs A) {
     }
 }
 
+// ------------------------------------------------- 2D, two sweeps per pass --
+// Temporal blocking for the 2D Dirichlet case (the reference's PERKS 2D
+// caches the slab across iterations, Stencil/jacobi2D/PERKS/
+// jacobi-general-kernel.cuh:22-505).  Each wavefront owns SEG = 64*VX columns
+// and streams a chunk of rows: at step r it computes sweep t+1 ("level 1") on
+// row r from level-0 rows r-1, r, r+1 and sweep t+2 on row r-1 from level-1
+// rows r-2, r-1, r, then stores only the latter.  Wavefronts are independent:
+// lane 0 and lane 63 also compute level 1 on the column just outside the
+// segment (from two extra level-0 columns per row), and every chunk
+// recomputes level 1 on the row just outside each end.  Same expressions as
+// one sweep, boundary points kept: bit-identical to two single sweeps.
+// columns per lane of the 2D two-sweep kernel: one 16-byte vector in
+// 1024-thread workgroups, two in 512-thread workgroups (room for the registers)
+template <typename T>
+constexpr int t2d_cols(int block) {
+    return (block == 512 ? 2 : 1) * 16 / (int)sizeof(T);
+}
+
+template <typename T, int VX, bool ALIGNED>
+__device__ __forceinline__ void store_row_nt(T *drow, int x, int nx, const T (&v)[VX], const bool (&xin)[VX]) {
+    constexpr int E = 16 / sizeof(T);
+    if (ALIGNED && x >= 1 && x + VX - 1 <= nx - 2) {
+#pragma unroll
+        for (int q = 0; q < VX / E; ++q) {
+            typename Vec16<T>::type t;
+#pragma unroll
+            for (int e = 0; e < E; ++e) t[e] = v[q * E + e];
+            store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E), t);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < VX; ++k)
+            if (xin[k]) drow[x + k] = v[k];
+    }
+}
+
+template <typename T, int VX, bool ALIGNED, int LA = 2>
+__device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
+                                           int tx, int yc, bool single, bool want0, bool want1, double &n0,
+                                           double &n1) {
+    constexpr int SEG = 64 * VX;
+    constexpr int NW = 3 + LA;  // level-0 rows r-1 .. r+1+LA (the last LA in flight)
+    const int lane = threadIdx.x & 63;
+    const int nx = A.nx, ny = A.ny;
+    const int x0 = tx * SEG;
+    const int x = x0 + lane * VX;
+    const int yb = 1 + yc * A.chunk_len;
+    const int ye = min(yb + A.chunk_len, ny - 1);
+    if (yb >= ye) return;  // uniform over the wavefront
+    const T quarter = T(0.25);
+    const bool lane_w = lane == 0, lane_e = lane == 63;
+    // edge columns of this lane: lane 0 -> x0-2, x0-1; lane 63 -> x0+SEG, x0+SEG+1
+    const int ex1 = lane_w ? x0 - 1 : x0 + SEG;      // adjacent column (level-1 edge column)
+    const int ex2 = lane_w ? x0 - 2 : x0 + SEG + 1;  // the one beyond
+    const bool edge_lane = lane_w || lane_e;
+    bool xin[VX];
+#pragma unroll
+    for (int k = 0; k < VX; ++k) xin[k] = x + k >= 1 && x + k <= nx - 2;
+    const bool ein = edge_lane && ex1 >= 1 && ex1 <= nx - 2;  // the level-1 edge column is interior
+
+    T a[NW][VX];       // level-0 rows
+    T e1[NW], e2[NW];  // level-0 edge columns ex1, ex2 of those rows (edge lanes)
+    T l[3][VX] = {};  // level-1 rows r-2, r-1, r
+    T le[3] = {};     // level-1 edge column ex1 of those rows (edge lanes)
+
+    auto load = [&](int y, T (&v)[VX], T &v1, T &v2) {
+        const bool ok = y >= 0 && y < ny;  // uniform
+        const T *pr = src + (size_t)(ok ? y : 0) * nx;
+        load_row<T, VX, ALIGNED>(pr, x, nx, ok, v);
+        v1 = (ok && edge_lane && ex1 >= 0 && ex1 < nx) ? pr[ex1] : T(0);
+        v2 = (ok && edge_lane && ex2 >= 0 && ex2 < nx) ? pr[ex2] : T(0);
+    };
+
+    const int r0 = single ? yb : yb - 1;
+    const int r1 = single ? ye - 1 : ye;  // inclusive
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) load(r0 - 1 + i, a[i], e1[i], e2[i]);
+    for (int r = r0; r <= r1; ++r) {
+        load(r + 1 + LA, a[NW - 1], e1[NW - 1], e2[NW - 1]);  // consumed LA steps later
+        const bool rin = r >= 1 && r <= ny - 2;
+        // ---- level 1 on row r (own columns, and the edge column for lanes 0 / 63)
+        {
+            const T *cr = a[1];
+            T west = __shfl_up(cr[VX - 1], 1, 64);
+            T east = __shfl_down(cr[0], 1, 64);
+            if (lane_w) west = e1[1];
+            if (lane_e) east = e1[1];
+            if (rin) {  // wave-uniform
+#pragma unroll
+                for (int k = 0; k < VX; ++k) {
+                    const T E = (k < VX - 1) ? cr[k + 1] : east;
+                    const T W = (k > 0) ? cr[k - 1] : west;
+                    const T v = quarter * (E + W + a[2][k] + a[0][k]);
+                    l[2][k] = xin[k] ? v : cr[k];
+                    if (want0 && xin[k] && r >= yb && r < ye) {
+                        const double d = (double)v - (double)cr[k];
+                        n0 += d * d;
+                    }
+                }
+                // edge column: lane 0 -> x0-1 (east neighbour x0 = cr[0]); lane 63 -> x0+SEG (west x0+SEG-1)
+                const T E = lane_w ? cr[0] : e2[1];
+                const T W = lane_w ? e2[1] : cr[VX - 1];
+                const T v = quarter * (E + W + e1[2] + e1[0]);
+                le[2] = ein ? v : e1[1];
+            } else {
+#pragma unroll
+                for (int k = 0; k < VX; ++k) l[2][k] = cr[k];
+                le[2] = e1[1];
+            }
+        }
+        if (single) {
+            if (rin) store_row_nt<T, VX, ALIGNED>(dst + (size_t)r * nx, x, nx, l[2], xin);
+        } else if (r - 1 >= yb && r - 1 >= 1 && r - 1 <= ny - 2) {
+            // ---- level 2 on row r-1
+            const T *cr = l[1];
+            T west = __shfl_up(cr[VX - 1], 1, 64);
+            T east = __shfl_down(cr[0], 1, 64);
+            if (lane_w) west = le[1];
+            if (lane_e) east = le[1];
+            T out[VX];
+#pragma unroll
+            for (int k = 0; k < VX; ++k) {
+                const T E = (k < VX - 1) ? cr[k + 1] : east;
+                const T W = (k > 0) ? cr[k - 1] : west;
+                const T v = quarter * (E + W + l[2][k] + l[0][k]);
+                out[k] = xin[k] ? v : cr[k];
+                if (want1 && xin[k]) {
+                    const double d = (double)v - (double)cr[k];
+                    n1 += d * d;
+                }
+            }
+            store_row_nt<T, VX, ALIGNED>(dst + (size_t)(r - 1) * nx, x, nx, out, xin);
+        }
+        // ---- slide
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) a[i][k] = a[i + 1][k];
+            e1[i] = e1[i + 1];
+            e2[i] = e2[i + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+            l[0][k] = l[1][k];
+            l[1][k] = l[2][k];
+        }
+        le[0] = le[1];
+        le[1] = le[2];
+    }
+}
+
+template <typename T, bool ALIGNED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
+    constexpr int VX = t2d_cols<T>(BLOCK);
+    __shared__ double lds[BLOCK / 64];
+    const int wave_id = xcd_wave_id();
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    const int nunits = A.ntx * A.chunks;
+    unsigned int epoch = 0;
+    int pass = 0;
+    for (int k = 0; k < A.niter; ++pass) {
+        const int steps = min(2, A.niter - k);
+        const int it = A.it0 + k;
+        const bool want0 = A.norm_every > 0 && (it % A.norm_every) == 0;
+        const bool want1 = steps == 2 && A.norm_every > 0 && ((it + 1) % A.norm_every) == 0;
+        const T *src = static_cast<const T *>((pass & 1) ? A.buf1 : A.buf0);
+        T *dst = static_cast<T *>((pass & 1) ? A.buf0 : A.buf1);
+        double n0 = 0.0, n1 = 0.0;
+        for (int u = wave_id; u < nunits; u += nwaves) {
+            const int yc = u / A.ntx;
+            const int tx = u - yc * A.ntx;
+            sweep2d_t2<T, VX, ALIGNED>(src, dst, A, tx, yc, steps == 1, want0, want1, n0, n1);
+        }
+        if (want0) {
+            const double s = block_sum_waves<BLOCK / 64>(n0, lds);
+            if (threadIdx.x == 0) A.norm_part[(size_t)(it / A.norm_every) * gridDim.x + blockIdx.x] = s;
+        }
+        if (want1) {
+            const double s = block_sum_waves<BLOCK / 64>(n1, lds);
+            if (threadIdx.x == 0) A.norm_part[(size_t)((it + 1) / A.norm_every) * gridDim.x + blockIdx.x] = s;
+        }
+        k += steps;
+        if (k < A.niter) {
+            if (!grid_barrier(A.sync, epoch)) return;
+        }
+    }
+}
+
 // ------------------------------------------------------------- dispatch ---
 using KernelFn = void (*)(JacobiArgs);
+
+static KernelFn pick_t2d_kernel(bool f64, bool aligned, int block) {
+    if (block == 512)
+        return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 512>
+                              : (KernelFn)jacobi2d_t2_kernel<double, false, 512>)
+                   : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 512>
+                              : (KernelFn)jacobi2d_t2_kernel<float, false, 512>);
+    return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 1024>
+                          : (KernelFn)jacobi2d_t2_kernel<double, false, 1024>)
+               : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 1024>
+                          : (KernelFn)jacobi2d_t2_kernel<float, false, 1024>);
+}
 
 // Default workgroup size per tile shape: one workgroup fills a CU (fewer
 // workgroups make the device-wide barrier cheaper).  3D: WX = 4 wavefronts
@@ -1103,17 +1303,27 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     const int vx = 16 / (int)pl->elem_size;
     const bool aligned = ((size_t)p->nx * pl->elem_size) % 16 == 0;
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
-    // two sweeps per pass: persistent 3D Dirichlet with one tile across x
-    const bool t2_ok = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
-                       p->nx <= kT2Width && p->tile_rows <= 0 &&
-                       (p->block_threads <= 0 || p->block_threads == 1024 || (f64 && p->block_threads == 512));
+    // two sweeps per pass: persistent Dirichlet; 3D with one tile across x
+    const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
+                        p->nx <= kT2Width && p->tile_rows <= 0 &&
+                        (p->block_threads <= 0 || p->block_threads == 1024 || (f64 && p->block_threads == 512));
+    const bool t2_ok2 = !is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
+                        p->tile_rows <= 0 &&
+                        (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
+    const bool t2_ok = t2_ok3 || t2_ok2;
     if (p->time_block == 2 && !t2_ok) {
         delete pl;
-        return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent 3D Dirichlet variant, "
-                                     "nx <= %d and the default tile shape", kT2Width);
+        return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent Dirichlet variant, "
+                                     "the default tile shape and, in 3D, nx <= %d", kT2Width);
     }
     pl->t2 = t2_ok && p->time_block != 1;
-    if (pl->t2) {
+    if (pl->t2 && !is3d) {
+        pl->rows = 1;
+        // 16384^2: FP64 537 GLUP/s at 512 threads (4 columns per lane) vs 516 at 1024;
+        // FP32 1196 at 1024 threads vs 942 at 512
+        pl->block = p->block_threads > 0 ? p->block_threads : (f64 ? 512 : 1024);
+        pl->kernel = pick_t2d_kernel(f64, aligned, pl->block);
+    } else if (pl->t2) {
         pl->rows = 1;
         // FP64: 4 columns per lane in 512-thread workgroups measured 465 vs 440 GLUP/s at 512^3
         pl->block = p->block_threads > 0 ? p->block_threads : (f64 ? 512 : 1024);
@@ -1148,8 +1358,9 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     const bool wg_tile = is3d && pl->block != 256;
     const int nworkers = wg_tile ? pl->grid : pl->grid * (pl->block / 64);
     int tile_w = wg_tile ? 64 * vx * 4 : 64 * vx;
+    if (pl->t2 && !is3d) tile_w = 64 * (f64 ? t2d_cols<double>(pl->block) : t2d_cols<float>(pl->block));
     int tile_h = wg_tile ? pl->rows * (pl->block / 256) : pl->rows;
-    if (pl->t2) {
+    if (pl->t2 && is3d) {
         tile_w = kT2Width;
         tile_h = f64 ? 4 : 8;
     }

@@ -174,11 +174,47 @@ def test_two_sweeps_per_pass_matches_one_sweep_per_pass(gpu):
     assert np.allclose(na, nb, rtol=NORM_RTOL, atol=0.0)
 
 
-@pytest.mark.parametrize("kw", [dict(bc=0), dict(bc=1, nx=600), dict(bc=1, variant=0), dict(bc=1, tile_rows=2)])
+T2_SHAPES_2D = [(3, 3), (5, 300), (35, 67), (130, 129), (64, 1030), (300, 256)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("block", [0, 512])
+@pytest.mark.parametrize("shape", T2_SHAPES_2D)
+@pytest.mark.parametrize("niter", [1, 2, 3, 8])
+def test_two_sweeps_per_pass_bitexact_2d(gpu, orc, dtype, block, shape, niter):
+    """2D temporal blocking: independent wavefronts that recompute the
+    intermediate sweep on their edge columns and chunk-end rows (one 16-byte
+    vector per lane in 1024-thread workgroups, two in 512-thread ones)."""
+    init = _rand(shape, dtype, 23 + niter)
+    _check(gpu, orc, init, niter, 1, time_block=2, block_threads=block)
+
+
+@pytest.mark.parametrize("norm_every", [1, 2, 3])
+@pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (4, 0), (0, 1), (9, 3), (400, 2)])
+@pytest.mark.parametrize("block", [0, 512])
+def test_two_sweeps_per_pass_norms_and_units_2d(gpu, orc, norm_every, chunks, grid, block):
+    init = _rand((421, 700), np.float64, 6)
+    _check(gpu, orc, init, 9, 1, norm_every=norm_every, time_block=2, chunks=chunks, grid_blocks=grid,
+           block_threads=block)
+
+
+def test_two_sweeps_per_pass_matches_one_sweep_per_pass_2d(gpu):
+    init = _rand((257, 2050), np.float32, 9)
+    a, na, ta = gpu.jacobi(init, 11, bc=1, norm_every=2, time_block=1)
+    b, nb, tb = gpu.jacobi(init, 11, bc=1, norm_every=2, time_block=2)
+    assert np.array_equal(a, b)
+    assert np.allclose(na, nb, rtol=NORM_RTOL, atol=0.0)
+    assert (ta["outer_iterations"], tb["outer_iterations"]) == (11, 6)
+
+
+@pytest.mark.parametrize("kw", [dict(bc=0), dict(bc=1, nx=600), dict(bc=1, variant=0), dict(bc=1, tile_rows=2),
+                                dict(bc=0, nz=1), dict(bc=1, nz=1, tile_rows=8), dict(bc=1, nz=1, variant=0),
+                                dict(bc=1, nz=1, block_threads=256)])
 def test_two_sweeps_per_pass_rejects_unsupported(gpu, kw):
     nx = kw.pop("nx", 64)
+    nz = kw.pop("nz", 8)
     bc = kw.pop("bc")
-    p = gpu.jacobi_params(nx, 16, 8, niter=2, bc=bc, time_block=2, **kw)
+    p = gpu.jacobi_params(nx, 16, nz, niter=2, bc=bc, time_block=2, **kw)
     with pytest.raises(gpu.CfmError) as e:
         gpu.JacobiPlan(p)
     assert e.value.status == gpu.ENOTSUP

@@ -42,7 +42,9 @@ void print_versions(const std::array<const char *, N> &names, size_t selected) {
 }
 
 inline int fail(const char *what, int st) {
-    std::fprintf(stderr, "ERROR: %s failed: %s (%d)\n", what, cfm_status_str(st), st);
+    const char *msg = cfm_last_error();
+    std::fprintf(stderr, "ERROR: %s failed: %s (%d)%s%s\n", what, cfm_status_str(st), st, msg && *msg ? ": " : "",
+                 msg ? msg : "");
     return 1;
 }
 

@@ -109,6 +109,23 @@ __device__ __forceinline__ bool grid_barrier(GridSync *s, unsigned int &epoch) {
     return s_abort == 0;
 }
 
+// Launches a persistent kernel (one that waits at grid_barrier).  A
+// cooperative launch guarantees that every workgroup is resident at once.
+// ROCm grants cooperative launches to one process at a time: while another
+// process holds that grant the launch fails with hipErrorIllegalState, and
+// the kernel then goes out as an ordinary launch.  Every grid here is sized to
+// the co-resident maximum of an otherwise idle device, so all workgroups still
+// start together; if one never became resident (another process filling the
+// CUs) the barrier watchdog ends the launch and it reports CFM_ETIMEOUT.
+inline hipError_t launch_persistent(const void *fn, int grid, int block, void **args, hipStream_t stream) {
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+    if (e == hipErrorIllegalState) {
+        (void)hipGetLastError();
+        e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+    }
+    return e;
+}
+
 // --------------------------------------------------------- reductions ----
 // xor butterfly over the 64 lanes of a wavefront: offsets 32,16,8,4,2,1.
 // Every lane ends with the same value because a+b == b+a exactly.

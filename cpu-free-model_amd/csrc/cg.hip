@@ -1006,8 +1006,8 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
         CFM_HIP(hipGetLastError());
         CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
         void *args[] = {&A};
-        CFM_HIP(hipLaunchCooperativeKernel((const void *)pl->K.persistent(pl->p.method), dim3(pl->grid),
-                                           dim3(kThreads * pl->sub), args, 0, pl->stream));
+        CFM_HIP(launch_persistent((const void *)pl->K.persistent(pl->p.method), pl->grid, kThreads * pl->sub, args,
+                                  pl->stream));
         CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
         CFM_HIP(hipEventSynchronize(pl->ev1));
         CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));

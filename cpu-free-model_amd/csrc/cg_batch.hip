@@ -803,8 +803,7 @@ static cfm_status batch_run(cfm_cg_batch_plan *pl, int max_iter, double tol, int
     CFM_HIP(hipGetLastError());
     CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
     void *args[] = {&A};
-    CFM_HIP(hipLaunchCooperativeKernel((const void *)batch_kernel(pl->sub), dim3(pl->grid), dim3(256 * pl->sub), args,
-                                       0, pl->stream));
+    CFM_HIP(launch_persistent((const void *)batch_kernel(pl->sub), pl->grid, 256 * pl->sub, args, pl->stream));
     CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
     CFM_HIP(hipEventSynchronize(pl->ev1));
     CFM_HIP(hipEventElapsedTime(ms, pl->ev0, pl->ev1));

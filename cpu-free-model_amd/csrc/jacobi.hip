@@ -98,6 +98,44 @@ __device__ __forceinline__ void st_vec(T *p, const T (&v)[VX]) {
     }
 }
 
+// x-neighbour exchange inside a wavefront: lane i receives lane i-1's value
+// (lane_up1) or lane i+1's (lane_dn1); lane 0 / lane 63 receive their own
+// value and are overwritten by the caller.  8-byte values move with DPP
+// whole-wavefront shifts (two v_mov_b32_dpp and no LDS traffic, instead of
+// two ds_bpermute): FP64 two-sweep kernels 469 -> 478 GLUP/s (3D 512^3) and
+// 537 -> 617 (2D 16384^2); 4-byte values measured no better, they keep
+// __shfl, as do the one-sweep kernels (DPP measured no faster there).
+template <typename T>
+__device__ __forceinline__ T lane_up1(T v) {
+    if constexpr (sizeof(T) == 8) {
+        union {
+            T t;
+            int i[2];
+        } a, r;
+        a.t = v;
+        r.i[0] = __builtin_amdgcn_update_dpp(a.i[0], a.i[0], 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+        r.i[1] = __builtin_amdgcn_update_dpp(a.i[1], a.i[1], 0x138, 0xf, 0xf, false);
+        return r.t;
+    } else {
+        return __shfl_up(v, 1, 64);
+    }
+}
+template <typename T>
+__device__ __forceinline__ T lane_dn1(T v) {
+    if constexpr (sizeof(T) == 8) {
+        union {
+            T t;
+            int i[2];
+        } a, r;
+        a.t = v;
+        r.i[0] = __builtin_amdgcn_update_dpp(a.i[0], a.i[0], 0x130 /* wave_shl:1 */, 0xf, 0xf, false);
+        r.i[1] = __builtin_amdgcn_update_dpp(a.i[1], a.i[1], 0x130, 0xf, 0xf, false);
+        return r.t;
+    } else {
+        return __shfl_down(v, 1, 64);
+    }
+}
+
 // Interior stores of the new field: it is not read again before the next
 // sweep, by which time it has left the caches, so it may bypass them.
 #ifndef CFM_JACOBI_NT_STORE
@@ -659,8 +697,8 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     // one level-1 row: out = stencil where interior, else the centre value
     auto l1row = [&](const typename Tl::Buf &B, int sl, int yy, bool pin, const T (&cr)[VX], const T (&N)[VX],
                      const T (&S)[VX], const T (&U)[VX], const T (&D)[VX], T (&out)[VX], bool norm) {
-        T west = __shfl_up(cr[VX - 1], 1, 64);
-        T east = __shfl_down(cr[0], 1, 64);
+        T west = lane_up1(cr[VX - 1]);
+        T east = lane_dn1(cr[0]);
         if (lane_w) west = wx > 0 ? B.l0e[wy][wx > 0 ? wx - 1 : 0][sl][1] : T(0);
         if (lane_e) east = wx < WX - 1 ? B.l0e[wy][wx < WX - 1 ? wx + 1 : 0][sl][0] : T(0);
         if (pin && yy >= 1 && yy <= ny - 2) {  // wave-uniform
@@ -669,7 +707,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
                 const T E = (k < VX - 1) ? cr[k + 1] : east;
                 const T W = (k > 0) ? cr[k - 1] : west;
                 const T v = c * (E + W + N[k] + S[k] + U[k] + D[k]);
-                out[k] = xin[k] ? v : cr[k];
+                out[k] = (FULL && k > 0 && k < VX - 1) ? v : (xin[k] ? v : cr[k]);  // FULL: inner columns are interior
                 if (norm && xin[k]) {
                     const double d = (double)v - (double)cr[k];
                     n0 += d * d;
@@ -752,8 +790,8 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
                     ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x], N);
                 }
                 const T *cr = l1o[1];
-                T west = __shfl_up(cr[VX - 1], 1, 64);
-                T east = __shfl_down(cr[0], 1, 64);
+                T west = lane_up1(cr[VX - 1]);
+                T east = lane_dn1(cr[0]);
                 if (lane_w) west = wx > 0 ? R.l1e[wy][wx > 0 ? wx - 1 : 0][1] : T(0);
                 if (lane_e) east = wx < WX - 1 ? R.l1e[wy][wx < WX - 1 ? wx + 1 : 0][0] : T(0);
                 T out[VX];
@@ -762,7 +800,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
                     const T E = (k < VX - 1) ? cr[k + 1] : east;
                     const T Wv = (k > 0) ? cr[k - 1] : west;
                     const T v = c * (E + Wv + N[k] + S[k] + l1o[2][k] + l1o[0][k]);
-                    out[k] = xin[k] ? v : cr[k];
+                    out[k] = (FULL && k > 0 && k < VX - 1) ? v : (xin[k] ? v : cr[k]);  // FULL: inner columns are interior
                     if (want1 && xin[k]) {
                         const double d = (double)v - (double)cr[k];
                         n1 += d * d;
@@ -1041,8 +1079,8 @@ __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restr
         // ---- level 1 on row r (own columns, and the edge column for lanes 0 / 63)
         {
             const T *cr = a[1];
-            T west = __shfl_up(cr[VX - 1], 1, 64);
-            T east = __shfl_down(cr[0], 1, 64);
+            T west = lane_up1(cr[VX - 1]);
+            T east = lane_dn1(cr[0]);
             if (lane_w) west = e1[1];
             if (lane_e) east = e1[1];
             if (rin) {  // wave-uniform
@@ -1073,8 +1111,8 @@ __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restr
         } else if (r - 1 >= yb && r - 1 >= 1 && r - 1 <= ny - 2) {
             // ---- level 2 on row r-1
             const T *cr = l[1];
-            T west = __shfl_up(cr[VX - 1], 1, 64);
-            T east = __shfl_down(cr[0], 1, 64);
+            T west = lane_up1(cr[VX - 1]);
+            T east = lane_dn1(cr[0]);
             if (lane_w) west = le[1];
             if (lane_e) east = le[1];
             T out[VX];
@@ -1319,9 +1357,9 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     pl->t2 = t2_ok && p->time_block != 1;
     if (pl->t2 && !is3d) {
         pl->rows = 1;
-        // 16384^2: FP64 537 GLUP/s at 512 threads (4 columns per lane) vs 516 at 1024;
-        // FP32 1196 at 1024 threads vs 942 at 512
-        pl->block = p->block_threads > 0 ? p->block_threads : (f64 ? 512 : 1024);
+        // 16384^2: FP64 617 GLUP/s at 1024 threads (2 columns per lane) vs 534 at 512;
+        // FP32 1205 at 1024 threads vs 942 at 512
+        pl->block = p->block_threads > 0 ? p->block_threads : 1024;
         pl->kernel = pick_t2d_kernel(f64, aligned, pl->block);
     } else if (pl->t2) {
         pl->rows = 1;
@@ -1453,8 +1491,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
             A.niter = p.niter;
             A.it0 = 0;
             void *args[] = {&A};
-            CFM_HIP(hipLaunchCooperativeKernel((const void *)pl->kernel, dim3(pl->grid), dim3(pl->block),
-                                               args, 0, pl->stream));
+            CFM_HIP(launch_persistent((const void *)pl->kernel, pl->grid, pl->block, args, pl->stream));
         }
     }
     CFM_HIP(hipEventRecord(pl->ev1, pl->stream));

@@ -63,6 +63,23 @@ def test_jacobi_versions_compare(gpu, name, shape, v):
 
 
 @pytest.mark.gpu
+def test_persistent_run_while_another_process_holds_cooperative_launch(gpu):
+    """ROCm grants cooperative launches to one process at a time: once this
+    process has made one, the CLI's persistent runs go out as ordinary
+    launches of the same co-resident grid and must still succeed."""
+    import numpy as np
+    out, _, _ = gpu.jacobi(np.ones((9, 70), dtype=np.float64), 3, variant=gpu.PERSISTENT)
+    assert np.all(out == 1.0)
+    for name, shape in (("jacobi2D", ("-nx", 130, "-ny", 67)), ("jacobi3D", ("-nx", 66, "-ny", 37, "-nz", 35))):
+        for v in (3, 5):
+            r = _run(name, "-v", v, *shape, "-niter", 9, "-compare")
+            assert r.returncode == 0, r.stderr
+    for v in (2, 3):  # Single Stream Standard / Pipelined: persistent CG
+        r = _run("cg", "-v", v, "-nx", 12, "-ny", 12, "-nz", 12, "-compare-single-gpu")
+        assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("extra", [("-bc", "dirichlet", "-init", "sin", "-tblock", 2),
                                    ("-bc", "dirichlet", "-prec", "f32", "-norm", 3),
                                    ("-bc", "periodic", "-norm", 2)])

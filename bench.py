@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 1 GiB streaming-copy probe (counter calibration) and exit")
+    ap.add_argument("--profile-cg", action="store_true",
+                    help="profiling run: the copy probe, then one 128^3 Poisson solve per CG method and "
+                         "schedule (one dispatch each); prints the iteration counts and exits")
     return ap.parse_args()
 
 
@@ -114,6 +117,28 @@ def cg_figures(cfm):
     out["ir_f32"]["rel_err_vs_f64"] = float(f"{np.linalg.norm(x32 - x64) / np.linalg.norm(x64):.3e}")
     out["ir_f32"]["true_rel_residual"] = float(f"{np.linalg.norm(b - M @ x32) / nb:.3e}")
     out["standard"]["true_rel_residual"] = float(f"{np.linalg.norm(b - M @ x64) / nb:.3e}")
+    return out
+
+
+def cg_profile_run(cfm):
+    """One solve per CG method on the 128^3 Poisson system, for counter runs:
+    the persistent kernels are one dispatch per solve, the discrete schedule one
+    dispatch per phase.  Returns the iterations and the matrix storage used."""
+    rp, col, val = cfm.poisson3d(128, 128, 128)
+    b = cfm.uniform(0, 0, rp.size - 1)
+    out = {"n": int(rp.size - 1), "nnz": int(rp[-1])}
+    for name, method, mode in (("standard", cfm.CG_STANDARD, cfm.CG_PERSISTENT),
+                               ("pipelined", cfm.CG_PIPELINED, cfm.CG_PERSISTENT),
+                               ("ir_f32", cfm.CG_IR_F32, cfm.CG_PERSISTENT),
+                               ("standard_discrete", cfm.CG_STANDARD, cfm.CG_DISCRETE),
+                               ("pipelined_discrete", cfm.CG_PIPELINED, cfm.CG_DISCRETE)):
+        p = cfm.cg_params(method=method, mode=mode, tol=1e-8, max_iter=10000)
+        with cfm.CgPlan(rp, col, val, p) as plan:
+            plan.set_rhs(b)
+            plan.set_x0(None)
+            st, hist, it, t = plan.solve()
+            out[name] = {"iterations": it, "outer": t["outer_iterations"], "ms": round(t["kernel_ms"], 3),
+                         "status": cfm.status_str(st)}
     return out
 
 
@@ -193,6 +218,11 @@ def main():
         sys.exit(2)
     import cfm
 
+    if args.profile_cg:
+        gbs, ms = cfm.bandwidth_probe(1 << 30, 5)
+        print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1)}), flush=True)
+        print(json.dumps(cg_profile_run(cfm)), flush=True)
+        return
     if args.copy_probe:
         gbs, ms = cfm.bandwidth_probe(1 << 30, 5)
         print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1), "best_ms": round(ms, 4)}))

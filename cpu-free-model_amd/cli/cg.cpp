@@ -99,6 +99,7 @@ bool read_matrix_market(const std::string &path, Csr &A) {
 
 int main(int argc, char *argv[]) {
     using namespace cli;
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);  // the report survives an abnormal end after main
     const std::array<const char *, 11> names{
         "Baseline Discrete Standard", "Baseline Discrete Pipelined", "Single Stream Standard",
         "Single Stream Pipelined", "Single Stream Pipelined Multi Overlap",

@@ -25,6 +25,9 @@
 namespace cli {
 
 inline int run_jacobi(int argc, char **argv, int dims) {
+    // line-buffered even into a file: the report survives a tool (a profiler)
+    // that ends the process abnormally after main returns
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);
     const std::array<const char *, 11> names{
         "Baseline Copy", "Baseline Overlap", "Baseline P2P", "Design 1", "Design 2", "PERKS",
         "Baseline Copy (No computation)", "Baseline Overlap (No Computation)",

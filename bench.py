@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 1 GiB streaming-copy probe (counter calibration) and exit")
+    ap.add_argument("--profile-batch", action="store_true",
+                    help="profiling run: the copy probe, then one batched solve of the 256-system "
+                         "variable-coefficient workload (default shape, one dispatch); prints its figures and exits")
     ap.add_argument("--profile-cg", action="store_true",
                     help="profiling run: the copy probe, then one 128^3 Poisson solve per CG method and "
                          "schedule (one dispatch each); prints the iteration counts and exits")
@@ -198,6 +201,28 @@ def cg_batch_figures(cfm, nsys, lmin=32, lmax=96, seed=2024):
             "mean_iters": round(float(iters.mean()), 1), "tuned_block": blk, "tuned_grid": grid}
 
 
+def cg_batch_profile_run(cfm, nsys, lmin=32, lmax=96, seed=2024):
+    """One batched solve of the cg_batch_figures workload with the default
+    workgroup shape (a single dispatch, for counter runs).  Returns the rows,
+    the stored entries and the per-system iteration counts' sum, from which the
+    algorithmic traffic of the solve follows."""
+    import numpy as np
+    systems = [cfm.varcoef3d(seed, sid, lmin, lmax)[:4] for sid in range(nsys)]
+    srp, rp, col, val = cfm.block_diag([s_[:3] for s_ in systems])
+    b = np.concatenate([s_[3] for s_ in systems])
+    p = cfm.cg_params(method=cfm.CG_PIPELINED, tol=1e-8, max_iter=20000)
+    with cfm.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve()
+    rows = np.diff(srp)
+    nnz = np.array([s_[0][-1] for s_ in systems], dtype=np.int64)
+    return {"systems": nsys, "rows": int(srp[-1]), "nnz": int(nnz.sum()), "status": cfm.status_str(st),
+            "ms": round(t["kernel_ms"], 3), "max_iters": int(iters.max()), "sum_iters": int(iters.sum()),
+            "row_iterations": int((rows * iters).sum()), "nnz_iterations": int((nnz * iters).sum()),
+            "grid": t["grid_blocks"], "block": t["block_threads"]}
+
+
 def warmstart_figures(nsys):
     """Learned warm start on held-out 64^3 variable-coefficient systems: one
     batched plan solved from x0 = 0 and from the trained network's guess
@@ -218,6 +243,11 @@ def main():
         sys.exit(2)
     import cfm
 
+    if args.profile_batch:
+        gbs, ms = cfm.bandwidth_probe(1 << 30, 5)
+        print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1)}), flush=True)
+        print(json.dumps(cg_batch_profile_run(cfm, args.batch_systems)), flush=True)
+        return
     if args.profile_cg:
         gbs, ms = cfm.bandwidth_probe(1 << 30, 5)
         print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1)}), flush=True)

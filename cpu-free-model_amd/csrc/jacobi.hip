@@ -1260,6 +1260,7 @@ static KernelFn pick_t2_kernel(bool aligned, bool full, int block) {
         if (block == 512) return pick_t2_shape<T, 2, 4, 4>(aligned, full);
         return pick_t2_shape<T, 4, 4, 2>(aligned, full);
     } else {
+        if (block == 512) return pick_t2_shape<T, 1, 8, 8>(aligned, full);
         return pick_t2_shape<T, 2, 8, 4>(aligned, full);
     }
 }
@@ -1344,7 +1345,7 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     // two sweeps per pass: persistent Dirichlet; 3D with one tile across x
     const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
                         p->nx <= kT2Width && p->tile_rows <= 0 &&
-                        (p->block_threads <= 0 || p->block_threads == 1024 || (f64 && p->block_threads == 512));
+                        (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
     const bool t2_ok2 = !is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
                         p->tile_rows <= 0 &&
                         (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
@@ -1363,8 +1364,10 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
         pl->kernel = pick_t2d_kernel(f64, aligned, pl->block);
     } else if (pl->t2) {
         pl->rows = 1;
-        // FP64: 4 columns per lane in 512-thread workgroups measured 465 vs 440 GLUP/s at 512^3
-        pl->block = p->block_threads > 0 ? p->block_threads : (f64 ? 512 : 1024);
+        // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428
+        // GLUP/s (2 columns, 1024 threads); FP32 one wavefront across x (8 columns per lane)
+        // 882 vs 844 (4 columns, 1024 threads)
+        pl->block = p->block_threads > 0 ? p->block_threads : 512;
         const bool full = p->nx == kT2Width;
         pl->kernel = f64 ? pick_t2_kernel<double>(aligned, full, pl->block)
                          : pick_t2_kernel<float>(aligned, full, pl->block);

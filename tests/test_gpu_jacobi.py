@@ -147,12 +147,14 @@ def test_full_size_2d_properties(gpu):
 T2_SHAPES = [(3, 5, 6), (9, 13, 512), (17, 23, 511), (40, 37, 100), (12, 70, 257)]
 
 
-@pytest.mark.parametrize("dtype,block", [(np.float64, 0), (np.float64, 512), (np.float32, 0)])
+@pytest.mark.parametrize("dtype,block", [(np.float64, 0), (np.float64, 512), (np.float64, 1024), (np.float32, 0),
+                                         (np.float32, 512)])
 @pytest.mark.parametrize("shape", T2_SHAPES)
 @pytest.mark.parametrize("niter", [1, 2, 3, 8])
 def test_two_sweeps_per_pass_bitexact(gpu, orc, dtype, block, shape, niter):
     """Temporal blocking (level 1 kept on chip) equals single sweeps bit for bit,
-    for both FP64 wavefront shapes (2 columns per lane x 16 waves, 4 x 8)."""
+    for every wavefront shape: FP64 2 columns per lane x 16 waves and 4 x 8,
+    FP32 4 x 16 and 8 x 8 (one wavefront across x)."""
     init = _rand(shape, dtype, 11 + niter)
     _check(gpu, orc, init, niter, 1, time_block=2, block_threads=block)
 

@@ -43,57 +43,79 @@ __device__ __forceinline__ int tap_off(int tap, int &dz, int &dy, int &dx) {
 
 // ------------------------------------------------------------- conv fwd ---
 // Y = conv3x3x3(X) + bias, optional ReLU.  W is [Cout][Kp] BF16 (k = tap*Cin+ci,
-// zero padded to Kp, a multiple of 32).  MT = Cout / 16 tiles per wavefront.
-template <int MT>
+// zero padded to Kp, a multiple of 32).  A wavefront computes NT tiles of 16
+// points x MT tiles of 16 output channels: per 32-wide k step it loads MT
+// weight fragments and NT gathered input fragments and issues MT*NT MFMAs, so
+// every fragment is reused NT (weights) or MT (inputs) times from registers.
+template <int MT, int NT>
 __global__ void __launch_bounds__(256) conv3_fwd_kernel(const bf16 *__restrict__ X, int Cin,
                                                         const bf16 *__restrict__ W, int Kp,
                                                         const float *__restrict__ bias, bf16 *__restrict__ Y,
                                                         float *__restrict__ Yf, int n, int npts, int relu) {
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int p0 = tile * 16;
+    const int p0 = tile * 16 * NT;
     if (p0 >= npts) return;  // uniform per wavefront
     const int nl = lane & 15, g = lane >> 4;
-    const int p = p0 + nl;
-    const bool pvalid = p < npts;  // npts need not be a multiple of 16
-    const int x = p % n, y = (p / n) % n, z = (p / (n * n)) % n, b = p / (n * n * n);
-    const int K = 27 * Cin;
-    f32x4 acc[MT];
+    int px[NT], py[NT], pz[NT], pb[NT];
+    bool pvalid[NT];  // npts need not be a multiple of 16
 #pragma unroll
-    for (int m = 0; m < MT; ++m) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < NT; ++t) {
+        const int p = p0 + 16 * t + nl;
+        pvalid[t] = p < npts;
+        px[t] = p % n;
+        py[t] = (p / n) % n;
+        pz[t] = (p / (n * n)) % n;
+        pb[t] = p / (n * n * n);
+    }
+    const int K = 27 * Cin;
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < Kp; kb += 32) {
         const int k = kb + 8 * g;
-        bf16x8 bv;
+        bf16x8 av[MT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bv[j] = (bf16)0.0f;
-        if (pvalid && k < K) {
-            const int tap = k / Cin, ci = k - tap * Cin;
-            int dz, dy, dx;
+        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8 *>(W + (size_t)(m * 16 + nl) * Kp + k);
+        const bool kin = k < K;
+        int tap = 0, ci = 0, dz = 0, dy = 0, dx = 0;
+        if (kin) {
+            tap = k / Cin;
+            ci = k - tap * Cin;
             tap_off(tap, dz, dy, dx);
-            const int zz = z + dz, yy = y + dy, xx = x + dx;
-            if (zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n) {
-                const size_t q = (((size_t)b * n + zz) * n + yy) * n + xx;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            bf16x8 bv;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bv[j] = (bf16)0.0f;
+            const int zz = pz[t] + dz, yy = py[t] + dy, xx = px[t] + dx;
+            if (kin && pvalid[t] && zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n) {
+                const size_t q = (((size_t)pb[t] * n + zz) * n + yy) * n + xx;
                 bv = *reinterpret_cast<const bf16x8 *>(X + q * Cin + ci);
             }
-        }
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const bf16x8 av = *reinterpret_cast<const bf16x8 *>(W + (size_t)(m * 16 + nl) * Kp + k);
-            acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[m], 0, 0, 0);
+            for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][t], 0, 0, 0);
         }
     }
-    // D[4g+i][nl]: output channel m*16 + 4g + i of point p0 + nl
+    // D[4g+i][nl]: output channel m*16 + 4g + i of point p0 + 16t + nl
     const int Cout = MT * 16;
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
+    for (int t = 0; t < NT; ++t) {
+        if (!pvalid[t]) continue;
+        const size_t p = (size_t)p0 + 16 * t + nl;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int co = m * 16 + 4 * g + i;
-            if (!pvalid) continue;
-            float v = acc[m][i] + (bias ? bias[co] : 0.f);
-            if (relu && v < 0.f) v = 0.f;
-            if (Y) Y[(size_t)p * Cout + co] = f2bf(v);
-            if (Yf) Yf[(size_t)p * Cout + co] = v;
+        for (int m = 0; m < MT; ++m) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = m * 16 + 4 * g + i;
+                float v = acc[m][t][i] + (bias ? bias[co] : 0.f);
+                if (relu && v < 0.f) v = 0.f;
+                if (Y) Y[p * Cout + co] = f2bf(v);
+                if (Yf) Yf[p * Cout + co] = v;
+            }
         }
     }
 }
@@ -127,14 +149,16 @@ __global__ void __launch_bounds__(256) conv3_wgrad_kernel(const bf16 *__restrict
     const int pbeg = s * slab, pend = min(npts, pbeg + slab);
     for (int pb = pbeg; pb < pend; pb += 32) {
         bf16x8 av, bv;
+        // coordinates of the lane's first point, then stepped along x with carries
+        const int p8 = pb + 8 * g;
+        int x = p8 % n, y = (p8 / n) % n, z = (p8 / (n * n)) % n, b = p8 / (n * n * n);
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int p = pb + 8 * g + j;
+            const int p = p8 + j;
             float a = 0.f, bb = 0.f;
             if (p < pend) {
                 a = bf2f(dZ[(size_t)p * Cout + co]);
                 if (kvalid) {
-                    const int x = p % n, y = (p / n) % n, z = (p / (n * n)) % n, b = p / (n * n * n);
                     const int zz = z + dz, yy = y + dy, xx = x + dx;
                     if (zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n)
                         bb = bf2f(X[((((size_t)b * n + zz) * n + yy) * n + xx) * Cin + ci]);
@@ -142,6 +166,16 @@ __global__ void __launch_bounds__(256) conv3_wgrad_kernel(const bf16 *__restrict
             }
             av[j] = f2bf(a);
             bv[j] = f2bf(bb);
+            if (++x == n) {
+                x = 0;
+                if (++y == n) {
+                    y = 0;
+                    if (++z == n) {
+                        z = 0;
+                        ++b;
+                    }
+                }
+            }
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
     }
@@ -161,14 +195,27 @@ __global__ void reduce_slabs_kernel(const float *__restrict__ partial, int nslab
     out[i] = a;
 }
 
-// per-slab channel sums of dZ (bias gradient), partial[slab][c]
-__global__ void chan_sum_kernel(const bf16 *__restrict__ dZ, int C, int npts, int slab, float *__restrict__ partial) {
-    const int s = blockIdx.x;
-    for (int c = threadIdx.x; c < C; c += blockDim.x) {
-        float a = 0.f;
-        const int pend = min(npts, (s + 1) * slab);
-        for (int p = s * slab; p < pend; ++p) a += bf2f(dZ[(size_t)p * C + c]);
-        partial[(size_t)s * C + c] = a;
+// per-slab channel sums of dZ (bias gradient), partial[slab][c].  256 threads
+// per slab: thread t < R*C (R = 256 / C rows at a time) owns channel t % C and
+// rows t / C, t / C + R, ... of the slab, so each pass reads 256 consecutive
+// values; the R row-partials of a channel are then added in row order.
+__global__ void __launch_bounds__(256) chan_sum_kernel(const bf16 *__restrict__ dZ, int C, int npts, int slab,
+                                                       float *__restrict__ partial) {
+    __shared__ float red[256];
+    const int s = blockIdx.x, t = threadIdx.x;
+    const int R = 256 / C;  // C <= 256
+    const int pbeg = s * slab, pend = min(npts, pbeg + slab);
+    float a = 0.f;
+    if (t < R * C) {
+        const int c = t % C;
+        for (int p = pbeg + t / C; p < pend; p += R) a += bf2f(dZ[(size_t)p * C + c]);
+    }
+    red[t] = a;
+    __syncthreads();
+    if (t < C) {
+        float v = 0.f;
+        for (int r = 0; r < R; ++r) v += red[r * C + t];
+        partial[(size_t)s * C + t] = v;
     }
 }
 
@@ -557,19 +604,35 @@ extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
     return CFM_OK;
 }
 
-// Y = conv(X); mt = Cout/16
+// Y = conv(X); mt = Cout/16 channel tiles, nt point tiles per wavefront
+template <int MT>
+static void launch_conv_fwd(int nt, size_t npts, hipStream_t st, const bf16 *X, int cin, const bf16 *W, int kp,
+                            const float *bias, bf16 *Y, int n, int relu) {
+    const size_t waves = (npts + 16 * nt - 1) / (16 * nt);
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    if (nt == 4)
+        hipLaunchKernelGGL((conv3_fwd_kernel<MT, 4>), grid, dim3(256), 0, st, X, cin, W, kp, bias, Y, nullptr, n,
+                           (int)npts, relu);
+    else if (nt == 2)
+        hipLaunchKernelGGL((conv3_fwd_kernel<MT, 2>), grid, dim3(256), 0, st, X, cin, W, kp, bias, Y, nullptr, n,
+                           (int)npts, relu);
+    else
+        hipLaunchKernelGGL((conv3_fwd_kernel<MT, 1>), grid, dim3(256), 0, st, X, cin, W, kp, bias, Y, nullptr, n,
+                           (int)npts, relu);
+}
+
 static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, int kp, const float *bias, bf16 *Y,
                            int cout, int level, int relu) {
     const int n = u->n >> level;
     const size_t npts = u->P[level];
-    const size_t tiles = (npts + 15) / 16;
-    const dim3 grid((unsigned)((tiles + 3) / 4));
+    // more point tiles per wavefront while the grid still has >= 16 wavefronts per CU
+    const int nt = npts >= (size_t)4096 * 64 ? 4 : npts >= (size_t)4096 * 32 ? 2 : 1;
     switch (cout / 16) {
-        case 1: hipLaunchKernelGGL(conv3_fwd_kernel<1>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
-        case 2: hipLaunchKernelGGL(conv3_fwd_kernel<2>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
-        case 3: hipLaunchKernelGGL(conv3_fwd_kernel<3>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
-        case 4: hipLaunchKernelGGL(conv3_fwd_kernel<4>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
-        case 6: hipLaunchKernelGGL(conv3_fwd_kernel<6>, grid, dim3(256), 0, u->stream, X, cin, W, kp, bias, Y, nullptr, n, (int)npts, relu); break;
+        case 1: launch_conv_fwd<1>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
+        case 2: launch_conv_fwd<2>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
+        case 3: launch_conv_fwd<3>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
+        case 4: launch_conv_fwd<4>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
+        case 6: launch_conv_fwd<6>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
         default: return CFM_EINVAL;
     }
     CFM_HIP(hipGetLastError());
@@ -647,7 +710,8 @@ static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) 
     hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->partial, nslab, count, u->gk);
     hipLaunchKernelGGL(unpack_wgrad_kernel, blocks_for((size_t)s.cout * s.cin_real * 27), dim3(256), 0, u->stream,
                        u->gk, s.cout, s.cin_real, s.cin, u->Kp[I], u->grad + u->woff[I]);
-    hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(64), 0, u->stream, dZ, s.cout, (int)npts, u->slab, u->partial);
+    hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(256), 0, u->stream, dZ, s.cout, (int)npts, u->slab,
+                       u->partial);
     hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(s.cout), dim3(256), 0, u->stream, u->partial, nslab,
                        (size_t)s.cout, u->grad + u->boff[I]);
     CFM_HIP(hipGetLastError());

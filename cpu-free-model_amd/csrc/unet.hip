@@ -120,6 +120,95 @@ __global__ void __launch_bounds__(256) conv3_fwd_kernel(const bf16 *__restrict__
     }
 }
 
+// LDS-staged forward conv for the layer shapes of the network.  A workgroup
+// owns 16 (x) x 4 (y) x TZ (z) output points of one batch element; the input
+// block with its one-point halo and all CIN channels is staged in LDS once
+// (zeros outside the grid), so a lane's B fragment (8 channels of one tap at
+// one point) is a single ds_read_b128 with no bounds checks.  Wavefront w
+// computes TZ == 2: rows y0 + 2(w&1) + {0,1} of plane z0 + (w>>1) (NT = 2);
+// TZ == 1: row y0 + w (NT = 1), times MT channel tiles.
+template <int MT, int CIN, int TZ>
+__global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
+                                                            int Kp, const float *__restrict__ bias,
+                                                            bf16 *__restrict__ Y, int n, int nbx, int nby, int nbz,
+                                                            int relu) {
+    constexpr int HX = 18, HY = 6, HZ = TZ + 2, NV = CIN / 8;
+    constexpr int NT = TZ == 2 ? 2 : 1;
+    static_assert(CIN % 8 == 0 && (TZ == 1 || TZ == 2), "shape");
+    __shared__ bf16x8 blk[HZ * HY * HX * NV];
+    int bid = blockIdx.x;
+    const int bx = bid % nbx;
+    bid /= nbx;
+    const int by = bid % nby;
+    bid /= nby;
+    const int bz = bid % nbz;
+    const int b = bid / nbz;
+    const int x0 = bx * 16, y0 = by * 4, z0 = bz * TZ;
+    const bf16x8 zero8 = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f,
+                          (bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+    for (int v = threadIdx.x; v < HZ * HY * HX * NV; v += 256) {
+        const int cv = v % NV;
+        int r = v / NV;
+        const int cx = r % HX;
+        r /= HX;
+        const int cy = r % HY, cz = r / HY;
+        const int gx = x0 - 1 + cx, gy = y0 - 1 + cy, gz = z0 - 1 + cz;
+        bf16x8 val = zero8;
+        if (gx >= 0 && gx < n && gy >= 0 && gy < n && gz >= 0 && gz < n)
+            val = *reinterpret_cast<const bf16x8 *>(X + ((((size_t)b * n + gz) * n + gy) * n + gx) * CIN + cv * 8);
+        blk[v] = val;
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nl = lane & 15, g = lane >> 4;
+    const int tz = TZ == 2 ? (w >> 1) : 0;
+    int ty[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) ty[t] = TZ == 2 ? 2 * (w & 1) + t : w;
+    constexpr int K = 27 * CIN;
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < Kp; kb += 32) {
+        const int k = kb + 8 * g;
+        bf16x8 av[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8 *>(W + (size_t)(m * 16 + nl) * Kp + k);
+        const bool kin = k < K;
+        const int tap = kin ? k / CIN : 26;  // the zero-padded tail of K reads a valid slot, then discards it
+        const int ci8 = kin ? (k % CIN) / 8 : 0;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // offsets + 1
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            bf16x8 bv = blk[(((tz + dz) * HY + (ty[t] + dy)) * HX + (nl + dx)) * NV + ci8];
+            if (!kin) bv = zero8;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][t], 0, 0, 0);
+        }
+    }
+    // D[4g+i][nl]: output channel m*16 + 4g + i of point (x0 + nl, y0 + ty[t], z0 + tz)
+    constexpr int Cout = MT * 16;
+    const int x = x0 + nl, z = z0 + tz;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int y = y0 + ty[t];
+        if (x >= n || y >= n || z >= n) continue;
+        const size_t p = (((size_t)b * n + z) * n + y) * n + x;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = m * 16 + 4 * g + i;
+                float v = acc[m][t][i] + (bias ? bias[co] : 0.f);
+                if (relu && v < 0.f) v = 0.f;
+                Y[p * Cout + co] = f2bf(v);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------ conv weight grad ---
 // partial[slab][co][k] = sum over points of slab: dZ[p][co] * Xs[p][k]
 // (M = 16 output channels, N = 16 k, K = 32 points per MFMA).  Each
@@ -220,85 +309,110 @@ __global__ void __launch_bounds__(256) chan_sum_kernel(const bf16 *__restrict__ 
 }
 
 // --------------------------------------------------------- elementwise ----
+// The elementwise kernels below move 8 channels (one 16-byte vector) per
+// thread; every channel count and offset of the network is a multiple of 8
+// and points number fewer than 2^27, so 32-bit index math suffices.
+__device__ __forceinline__ bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+__device__ __forceinline__ void st8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
+
 __global__ void relu_bwd_kernel(const bf16 *__restrict__ Y, const bf16 *__restrict__ dY, bf16 *__restrict__ dZ,
                                 size_t count) {
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i < count) dZ[i] = bf2f(Y[i]) > 0.f ? dY[i] : (bf16)0.0f;
+    const size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 8;
+    if (i >= count) return;
+    const bf16x8 y = ld8(Y + i), d = ld8(dY + i);
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = bf2f(y[j]) > 0.f ? d[j] : (bf16)0.0f;
+    st8(dZ + i, o);
 }
 
 // 2x2x2 average pooling: n -> n/2
 __global__ void pool_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y, int C, int n, int nb) {
-    const int m = n / 2;
-    const size_t total = (size_t)nb * m * m * m * C;
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = i % C;
-    size_t q = i / C;
-    const int x = q % m, y = (q / m) % m, z = (q / ((size_t)m * m)) % m, b = q / ((size_t)m * m * m);
-    float a = 0.f;
+    const unsigned m = n / 2, V = C / 8;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)nb * m * m * m * V) return;
+    const unsigned cv = i % V, q = i / V;
+    const unsigned x = q % m, y = (q / m) % m, z = (q / (m * m)) % m, b = q / (m * m * m);
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int dz = 0; dz < 2; ++dz)
         for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx)
-                a += bf2f(X[((((size_t)b * n + 2 * z + dz) * n + 2 * y + dy) * n + 2 * x + dx) * C + c]);
-    Y[i] = f2bf(a * 0.125f);
+            for (int dx = 0; dx < 2; ++dx) {
+                const bf16x8 v = ld8(X + ((((size_t)b * n + 2 * z + dz) * n + 2 * y + dy) * n + 2 * x + dx) * C + 8 * cv);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] += bf2f(v[j]);
+            }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = f2bf(a[j] * 0.125f);
+    st8(Y + (size_t)q * C + 8 * cv, o);
 }
 
 __global__ void pool_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int n, int nb) {
-    const size_t total = (size_t)nb * n * n * n * C;
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = i % C;
-    size_t q = i / C;
-    const int m = n / 2;
-    const int x = q % n, y = (q / n) % n, z = (q / ((size_t)n * n)) % n, b = q / ((size_t)n * n * n);
-    dX[i] = f2bf(bf2f(dY[((((size_t)b * m + z / 2) * m + y / 2) * m + x / 2) * C + c]) * 0.125f);
+    const unsigned V = C / 8, un = n, m = n / 2;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)nb * un * un * un * V) return;
+    const unsigned cv = i % V, q = i / V;
+    const unsigned x = q % un, y = (q / un) % un, z = (q / (un * un)) % un, b = q / (un * un * un);
+    const bf16x8 v = ld8(dY + ((((size_t)b * m + z / 2) * m + y / 2) * m + x / 2) * C + 8 * cv);
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = f2bf(bf2f(v[j]) * 0.125f);
+    st8(dX + (size_t)q * C + 8 * cv, o);
 }
 
 // nearest upsampling m -> 2m, written into channels [off, off+C) of a buffer with Ct channels
 __global__ void up_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y, int C, int Ct, int off, int m,
                               int nb) {
-    const int n = 2 * m;
-    const size_t total = (size_t)nb * n * n * n * C;
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = i % C;
-    size_t q = i / C;
-    const int x = q % n, y = (q / n) % n, z = (q / ((size_t)n * n)) % n, b = q / ((size_t)n * n * n);
-    Y[q * Ct + off + c] = X[((((size_t)b * m + z / 2) * m + y / 2) * m + x / 2) * C + c];
+    const unsigned n = 2 * m, V = C / 8, um = m;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)nb * n * n * n * V) return;
+    const unsigned cv = i % V, q = i / V;
+    const unsigned x = q % n, y = (q / n) % n, z = (q / (n * n)) % n, b = q / (n * n * n);
+    st8(Y + (size_t)q * Ct + off + 8 * cv, ld8(X + ((((size_t)b * um + z / 2) * um + y / 2) * um + x / 2) * C + 8 * cv));
 }
 
 // gradient of nearest upsampling: sum of the 8 children (from channels [off, off+C) of dY)
 __global__ void up_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int Ct, int off, int m,
                               int nb) {
-    const int n = 2 * m;
-    const size_t total = (size_t)nb * m * m * m * C;
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int c = i % C;
-    size_t q = i / C;
-    const int x = q % m, y = (q / m) % m, z = (q / ((size_t)m * m)) % m, b = q / ((size_t)m * m * m);
-    float a = 0.f;
+    const unsigned n = 2 * m, V = C / 8, um = m;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)nb * um * um * um * V) return;
+    const unsigned cv = i % V, q = i / V;
+    const unsigned x = q % um, y = (q / um) % um, z = (q / (um * um)) % um, b = q / (um * um * um);
+    float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int dz = 0; dz < 2; ++dz)
         for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx)
-                a += bf2f(dY[((((size_t)b * n + 2 * z + dz) * n + 2 * y + dy) * n + 2 * x + dx) * Ct + off + c]);
-    dX[i] = f2bf(a);
+            for (int dx = 0; dx < 2; ++dx) {
+                const bf16x8 v =
+                    ld8(dY + ((((size_t)b * n + 2 * z + dz) * n + 2 * y + dy) * n + 2 * x + dx) * Ct + off + 8 * cv);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] += bf2f(v[j]);
+            }
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = f2bf(a[j]);
+    st8(dX + (size_t)q * C + 8 * cv, o);
 }
 
 // copy C channels between buffers with different channel counts / offsets
 __global__ void chan_copy_kernel(const bf16 *__restrict__ src, int Cs, int offs, bf16 *__restrict__ dst, int Cd,
                                  int offd, int C, size_t npts) {
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= npts * C) return;
-    const size_t p = i / C;
-    const int c = i % C;
-    dst[p * Cd + offd + c] = src[p * Cs + offs + c];
+    const unsigned V = C / 8;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (unsigned)npts * V) return;
+    const unsigned p = i / V, cv = i % V;
+    st8(dst + (size_t)p * Cd + offd + 8 * cv, ld8(src + (size_t)p * Cs + offs + 8 * cv));
 }
 
 // add two gradient contributions: a += b (elementwise)
 __global__ void add_kernel(bf16 *__restrict__ a, const bf16 *__restrict__ b, size_t count) {
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i < count) a[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+    const size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 8;
+    if (i >= count) return;
+    const bf16x8 x = ld8(a + i), y = ld8(b + i);
+    bf16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o[j] = f2bf(bf2f(x[j]) + bf2f(y[j]));
+    st8(a + i, o);
 }
 
 // input packing: host-order float planes [b][2][n^3] -> NDHWC BF16 with 8 channels (2 used)
@@ -621,10 +735,48 @@ static void launch_conv_fwd(int nt, size_t npts, hipStream_t st, const bf16 *X, 
                            (int)npts, relu);
 }
 
+template <int MT, int CIN>
+static void launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
+                            const float *bias, bf16 *Y, int relu) {
+    constexpr int TZ = CIN * 2 * 18 * 6 * 4 <= 65536 ? 2 : 1;  // the staged block fits 64 KiB of LDS
+    const int nbx = (n + 15) / 16, nby = (n + 3) / 4, nbz = (n + TZ - 1) / TZ;
+    const dim3 grid((unsigned)(nbx * nby * nbz * batch));
+    hipLaunchKernelGGL((conv3_fwd_lds_kernel<MT, CIN, TZ>), grid, dim3(256), 0, st, X, W, kp, bias, Y, n, nbx, nby,
+                       nbz, relu);
+}
+
+// the (input channels, output channel tiles) pairs of the network's forward and data-gradient convs
+static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
+                     const float *bias, bf16 *Y, int relu) {
+#define CFM_CONV_LDS(C, M)                                                  \
+    if (cin == C && mt == M) {                                              \
+        launch_conv_lds<M, C>(n, batch, st, X, W, kp, bias, Y, relu);       \
+        return true;                                                        \
+    }
+    CFM_CONV_LDS(8, 1)
+    CFM_CONV_LDS(16, 1)
+    CFM_CONV_LDS(16, 2)
+    CFM_CONV_LDS(16, 3)
+    CFM_CONV_LDS(32, 1)
+    CFM_CONV_LDS(32, 2)
+    CFM_CONV_LDS(32, 4)
+    CFM_CONV_LDS(32, 6)
+    CFM_CONV_LDS(48, 1)
+    CFM_CONV_LDS(64, 2)
+    CFM_CONV_LDS(64, 4)
+    CFM_CONV_LDS(96, 2)
+#undef CFM_CONV_LDS
+    return false;
+}
+
 static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, int kp, const float *bias, bf16 *Y,
                            int cout, int level, int relu) {
     const int n = u->n >> level;
     const size_t npts = u->P[level];
+    if (cout % 16 == 0 && conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu)) {
+        CFM_HIP(hipGetLastError());
+        return CFM_OK;
+    }
     // more point tiles per wavefront while the grid still has >= 16 wavefronts per CU
     const int nt = npts >= (size_t)4096 * 64 ? 4 : npts >= (size_t)4096 * 32 ? 2 : 1;
     switch (cout / 16) {
@@ -651,18 +803,18 @@ static cfm_status run_forward_dev(cfm_unet *u, const float *d_in) {
     if ((st = conv_fwd(u, X, kConv[I].cin, u->Wf[I], u->Kp[I], b[I], Y, kConv[I].cout, kConv[I].level, 1)) != CFM_OK) return st
     CONV(0, u->X0, u->A0);
     CONV(1, u->A0, u->S1);
-    hipLaunchKernelGGL(pool_fwd_kernel, blocks_for(P2 * 16), dim3(256), 0, u->stream, u->S1, u->Q1, 16, n, u->batch);
+    hipLaunchKernelGGL(pool_fwd_kernel, blocks_for(P2 * 16 / 8), dim3(256), 0, u->stream, u->S1, u->Q1, 16, n, u->batch);
     CONV(2, u->Q1, u->A2);
     CONV(3, u->A2, u->S2);
-    hipLaunchKernelGGL(pool_fwd_kernel, blocks_for(u->P[2] * 32), dim3(256), 0, u->stream, u->S2, u->Q2, 32, n / 2, u->batch);
+    hipLaunchKernelGGL(pool_fwd_kernel, blocks_for(u->P[2] * 32 / 8), dim3(256), 0, u->stream, u->S2, u->Q2, 32, n / 2, u->batch);
     CONV(4, u->Q2, u->A4);
     CONV(5, u->A4, u->A5);
-    hipLaunchKernelGGL(up_fwd_kernel, blocks_for(P2 * 64), dim3(256), 0, u->stream, u->A5, u->U2, 64, 96, 0, n / 4, u->batch);
-    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->S2, 32, 0, u->U2, 96, 64, 32, P2);
+    hipLaunchKernelGGL(up_fwd_kernel, blocks_for(P2 * 64 / 8), dim3(256), 0, u->stream, u->A5, u->U2, 64, 96, 0, n / 4, u->batch);
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->S2, 32, 0, u->U2, 96, 64, 32, P2);
     CONV(6, u->U2, u->A6);
     CONV(7, u->A6, u->A7);
-    hipLaunchKernelGGL(up_fwd_kernel, blocks_for(P1 * 32), dim3(256), 0, u->stream, u->A7, u->U1, 32, 48, 0, n / 2, u->batch);
-    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->S1, 16, 0, u->U1, 48, 32, 16, P1);
+    hipLaunchKernelGGL(up_fwd_kernel, blocks_for(P1 * 32 / 8), dim3(256), 0, u->stream, u->A7, u->U1, 32, 48, 0, n / 2, u->batch);
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->S1, 16, 0, u->U1, 48, 32, 16, P1);
     CONV(8, u->U1, u->A8);
     CONV(9, u->A8, u->A9);
 #undef CONV
@@ -725,7 +877,7 @@ static cfm_status conv_dgrad(cfm_unet *u, int I, const bf16 *dZ, bf16 *dX) {
 }
 
 static cfm_status relu_bwd(cfm_unet *u, const bf16 *Y, const bf16 *dY, bf16 *dZ, size_t count) {
-    hipLaunchKernelGGL(relu_bwd_kernel, blocks_for(count), dim3(256), 0, u->stream, Y, dY, dZ, count);
+    hipLaunchKernelGGL(relu_bwd_kernel, blocks_for(count / 8), dim3(256), 0, u->stream, Y, dY, dZ, count);
     CFM_HIP(hipGetLastError());
     return CFM_OK;
 }
@@ -752,8 +904,8 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     CK(relu_bwd(u, u->A8, u->g1a, u->g1b, P1 * 16));   // dZ8
     CK(conv_wgrad(u, 8, u->U1, u->g1b));
     CK(conv_dgrad(u, 8, u->g1b, u->g1c));               // dU1 [P1][48]
-    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g1c, u->g2a, 32, 48, 0, n / 2, u->batch);  // dA7
-    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->g1c, 48, 32, u->g1w, 16, 0, 16, P1);  // dS1 (skip)
+    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g1c, u->g2a, 32, 48, 0, n / 2, u->batch);  // dA7
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g1c, 48, 32, u->g1w, 16, 0, 16, P1);  // dS1 (skip)
     // decoder level 2
     CK(relu_bwd(u, u->A7, u->g2a, u->g2b, P2 * 32));   // dZ7
     CK(conv_wgrad(u, 7, u->A6, u->g2b));
@@ -761,8 +913,8 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     CK(relu_bwd(u, u->A6, u->g2a, u->g2b, P2 * 32));   // dZ6
     CK(conv_wgrad(u, 6, u->U2, u->g2b));
     CK(conv_dgrad(u, 6, u->g2b, u->g2c));               // dU2 [P2][96]
-    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P3 * 64), dim3(256), 0, u->stream, u->g2c, u->g3a, 64, 96, 0, n / 4, u->batch);  // dA5
-    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g2c, 96, 64, u->g2w, 32, 0, 32, P2);  // dS2 (skip)
+    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P3 * 64 / 8), dim3(256), 0, u->stream, u->g2c, u->g3a, 64, 96, 0, n / 4, u->batch);  // dA5
+    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g2c, 96, 64, u->g2w, 32, 0, 32, P2);  // dS2 (skip)
     // bottleneck
     CK(relu_bwd(u, u->A5, u->g3a, u->g3b, P3 * 64));   // dZ5
     CK(conv_wgrad(u, 5, u->A4, u->g3b));
@@ -770,8 +922,8 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     CK(relu_bwd(u, u->A4, u->g3a, u->g3b, P3 * 64));   // dZ4
     CK(conv_wgrad(u, 4, u->Q2, u->g3b));
     CK(conv_dgrad(u, 4, u->g3b, u->g3a));               // dQ2 [P3][32]
-    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g3a, u->g2a, 32, n / 2, u->batch);
-    hipLaunchKernelGGL(add_kernel, blocks_for(P2 * 32), dim3(256), 0, u->stream, u->g2a, u->g2w, P2 * 32);  // dS2
+    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g3a, u->g2a, 32, n / 2, u->batch);
+    hipLaunchKernelGGL(add_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g2a, u->g2w, P2 * 32);  // dS2
     // encoder level 2
     CK(relu_bwd(u, u->S2, u->g2a, u->g2b, P2 * 32));   // dZ3
     CK(conv_wgrad(u, 3, u->A2, u->g2b));
@@ -779,8 +931,8 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     CK(relu_bwd(u, u->A2, u->g2a, u->g2b, P2 * 32));   // dZ2
     CK(conv_wgrad(u, 2, u->Q1, u->g2b));
     CK(conv_dgrad(u, 2, u->g2b, u->g2a));               // dQ1 [P2][16]
-    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->g2a, u->g1a, 16, n, u->batch);
-    hipLaunchKernelGGL(add_kernel, blocks_for(P1 * 16), dim3(256), 0, u->stream, u->g1a, u->g1w, P1 * 16);  // dS1
+    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g2a, u->g1a, 16, n, u->batch);
+    hipLaunchKernelGGL(add_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g1a, u->g1w, P1 * 16);  // dS1
     // encoder level 1
     CK(relu_bwd(u, u->S1, u->g1a, u->g1b, P1 * 16));   // dZ1
     CK(conv_wgrad(u, 1, u->A0, u->g1b));

@@ -244,17 +244,17 @@ __global__ void __launch_bounds__(256) conv3_wgrad_kernel(const bf16 *__restrict
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int p = p8 + j;
-            float a = 0.f, bb = 0.f;
+            bf16 a = (bf16)0.0f, bb = (bf16)0.0f;
             if (p < pend) {
-                a = bf2f(dZ[(size_t)p * Cout + co]);
+                a = dZ[(size_t)p * Cout + co];
                 if (kvalid) {
                     const int zz = z + dz, yy = y + dy, xx = x + dx;
                     if (zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n)
-                        bb = bf2f(X[((((size_t)b * n + zz) * n + yy) * n + xx) * Cin + ci]);
+                        bb = X[((((size_t)b * n + zz) * n + yy) * n + xx) * Cin + ci];
                 }
             }
-            av[j] = f2bf(a);
-            bv[j] = f2bf(bb);
+            av[j] = a;
+            bv[j] = bb;
             if (++x == n) {
                 x = 0;
                 if (++y == n) {
@@ -274,13 +274,23 @@ __global__ void __launch_bounds__(256) conv3_wgrad_kernel(const bf16 *__restrict
     for (int i = 0; i < 4; ++i) out[(size_t)(mt * 16 + 4 * g + i) * Kp + kt * 16 + nl] = acc[i];
 }
 
-// out[i] = sum_s partial[s][i] (fixed order), i < count
+// out[i] = sum_s partial[s][i] (fixed order s = 0, 1, ...), i < count.  Eight
+// loads are issued before their eight in-order adds, so the memory latency is
+// paid once per eight slabs instead of once per slab.
 __global__ void reduce_slabs_kernel(const float *__restrict__ partial, int nslab, size_t count,
                                     float *__restrict__ out) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= count) return;
     float a = 0.f;
-    for (int s = 0; s < nslab; ++s) a += partial[(size_t)s * count + i];
+    int s = 0;
+    for (; s + 8 <= nslab; s += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = partial[(size_t)(s + j) * count + i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a += v[j];
+    }
+    for (; s < nslab; ++s) a += partial[(size_t)s * count + i];
     out[i] = a;
 }
 
@@ -297,7 +307,15 @@ __global__ void __launch_bounds__(256) chan_sum_kernel(const bf16 *__restrict__ 
     float a = 0.f;
     if (t < R * C) {
         const int c = t % C;
-        for (int p = pbeg + t / C; p < pend; p += R) a += bf2f(dZ[(size_t)p * C + c]);
+        int p = pbeg + t / C;
+        for (; p + 7 * R < pend; p += 8 * R) {  // eight loads in flight, adds in row order
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = bf2f(dZ[(size_t)(p + j * R) * C + c]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a += v[j];
+        }
+        for (; p < pend; p += R) a += bf2f(dZ[(size_t)p * C + c]);
     }
     red[t] = a;
     __syncthreads();
@@ -852,17 +870,18 @@ static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) 
     const ConvSpec &s = kConv[I];
     const int level = s.level, n = u->n >> level;
     const size_t npts = u->P[level];
-    const int nslab = (int)((npts + u->slab - 1) / u->slab);
+    const size_t slab = u->slab;  // fixed 4096-point slabs measured fastest (more wavefronts)
+    const int nslab = (int)((npts + slab - 1) / slab);
     const int ntile_k = u->Kp[I] / 16;
     const int ntile = (s.cout / 16) * ntile_k;
     const size_t count = (size_t)s.cout * u->Kp[I];
     hipLaunchKernelGGL(conv3_wgrad_kernel, dim3((ntile + 3) / 4, nslab), dim3(256), 0, u->stream, X, s.cin, dZ, s.cout,
-                       u->Kp[I], u->partial, n, (int)npts, u->slab, ntile_k);
+                       u->Kp[I], u->partial, n, (int)npts, (int)slab, ntile_k);
     CFM_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->partial, nslab, count, u->gk);
     hipLaunchKernelGGL(unpack_wgrad_kernel, blocks_for((size_t)s.cout * s.cin_real * 27), dim3(256), 0, u->stream,
                        u->gk, s.cout, s.cin_real, s.cin, u->Kp[I], u->grad + u->woff[I]);
-    hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(256), 0, u->stream, dZ, s.cout, (int)npts, u->slab,
+    hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(256), 0, u->stream, dZ, s.cout, (int)npts, (int)slab,
                        u->partial);
     hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(s.cout), dim3(256), 0, u->stream, u->partial, nslab,
                        (size_t)s.cout, u->grad + u->boff[I]);

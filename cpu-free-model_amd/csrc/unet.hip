@@ -10,7 +10,8 @@
 // k; lane l holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and
 // D[4(l/16)+i][l%16] (layout checked on the device).  Channel counts are
 // multiples of 8, so a lane's 8 k-values are 8 consecutive channels of one
-// tap: one 16-byte load.
+// tap: one 16-byte read from the workgroup's input block, which is staged in
+// LDS with its one-point halo.
 //
 // Backward data is the same kernel with the weights flipped in space and
 // transposed in channels; backward weights is a split-K MFMA GEMM over point
@@ -43,83 +44,7 @@ __device__ __forceinline__ int tap_off(int tap, int &dz, int &dy, int &dx) {
 
 // ------------------------------------------------------------- conv fwd ---
 // Y = conv3x3x3(X) + bias, optional ReLU.  W is [Cout][Kp] BF16 (k = tap*Cin+ci,
-// zero padded to Kp, a multiple of 32).  A wavefront computes NT tiles of 16
-// points x MT tiles of 16 output channels: per 32-wide k step it loads MT
-// weight fragments and NT gathered input fragments and issues MT*NT MFMAs, so
-// every fragment is reused NT (weights) or MT (inputs) times from registers.
-template <int MT, int NT>
-__global__ void __launch_bounds__(256) conv3_fwd_kernel(const bf16 *__restrict__ X, int Cin,
-                                                        const bf16 *__restrict__ W, int Kp,
-                                                        const float *__restrict__ bias, bf16 *__restrict__ Y,
-                                                        float *__restrict__ Yf, int n, int npts, int relu) {
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int p0 = tile * 16 * NT;
-    if (p0 >= npts) return;  // uniform per wavefront
-    const int nl = lane & 15, g = lane >> 4;
-    int px[NT], py[NT], pz[NT], pb[NT];
-    bool pvalid[NT];  // npts need not be a multiple of 16
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int p = p0 + 16 * t + nl;
-        pvalid[t] = p < npts;
-        px[t] = p % n;
-        py[t] = (p / n) % n;
-        pz[t] = (p / (n * n)) % n;
-        pb[t] = p / (n * n * n);
-    }
-    const int K = 27 * Cin;
-    f32x4 acc[MT][NT];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < Kp; kb += 32) {
-        const int k = kb + 8 * g;
-        bf16x8 av[MT];
-#pragma unroll
-        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8 *>(W + (size_t)(m * 16 + nl) * Kp + k);
-        const bool kin = k < K;
-        int tap = 0, ci = 0, dz = 0, dy = 0, dx = 0;
-        if (kin) {
-            tap = k / Cin;
-            ci = k - tap * Cin;
-            tap_off(tap, dz, dy, dx);
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            bf16x8 bv;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) bv[j] = (bf16)0.0f;
-            const int zz = pz[t] + dz, yy = py[t] + dy, xx = px[t] + dx;
-            if (kin && pvalid[t] && zz >= 0 && zz < n && yy >= 0 && yy < n && xx >= 0 && xx < n) {
-                const size_t q = (((size_t)pb[t] * n + zz) * n + yy) * n + xx;
-                bv = *reinterpret_cast<const bf16x8 *>(X + q * Cin + ci);
-            }
-#pragma unroll
-            for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][t], 0, 0, 0);
-        }
-    }
-    // D[4g+i][nl]: output channel m*16 + 4g + i of point p0 + 16t + nl
-    const int Cout = MT * 16;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        if (!pvalid[t]) continue;
-        const size_t p = (size_t)p0 + 16 * t + nl;
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int co = m * 16 + 4 * g + i;
-                float v = acc[m][t][i] + (bias ? bias[co] : 0.f);
-                if (relu && v < 0.f) v = 0.f;
-                if (Y) Y[p * Cout + co] = f2bf(v);
-                if (Yf) Yf[p * Cout + co] = v;
-            }
-        }
-    }
-}
-
+// zero padded to Kp, a multiple of 32).
 // LDS-staged forward conv for the layer shapes of the network.  A workgroup
 // owns 16 (x) x 4 (y) x TZ (z) output points of one batch element; the input
 // block with its one-point halo and all CIN channels is staged in LDS once
@@ -736,23 +661,6 @@ extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
     return CFM_OK;
 }
 
-// Y = conv(X); mt = Cout/16 channel tiles, nt point tiles per wavefront
-template <int MT>
-static void launch_conv_fwd(int nt, size_t npts, hipStream_t st, const bf16 *X, int cin, const bf16 *W, int kp,
-                            const float *bias, bf16 *Y, int n, int relu) {
-    const size_t waves = (npts + 16 * nt - 1) / (16 * nt);
-    const dim3 grid((unsigned)((waves + 3) / 4));
-    if (nt == 4)
-        hipLaunchKernelGGL((conv3_fwd_kernel<MT, 4>), grid, dim3(256), 0, st, X, cin, W, kp, bias, Y, nullptr, n,
-                           (int)npts, relu);
-    else if (nt == 2)
-        hipLaunchKernelGGL((conv3_fwd_kernel<MT, 2>), grid, dim3(256), 0, st, X, cin, W, kp, bias, Y, nullptr, n,
-                           (int)npts, relu);
-    else
-        hipLaunchKernelGGL((conv3_fwd_kernel<MT, 1>), grid, dim3(256), 0, st, X, cin, W, kp, bias, Y, nullptr, n,
-                           (int)npts, relu);
-}
-
 template <int MT, int CIN>
 static void launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
                             const float *bias, bf16 *Y, int relu) {
@@ -787,24 +695,12 @@ static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf
     return false;
 }
 
+// Y = conv(X) for the (input channels, output channels) pairs of the network
 static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, int kp, const float *bias, bf16 *Y,
                            int cout, int level, int relu) {
     const int n = u->n >> level;
-    const size_t npts = u->P[level];
-    if (cout % 16 == 0 && conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu)) {
-        CFM_HIP(hipGetLastError());
-        return CFM_OK;
-    }
-    // more point tiles per wavefront while the grid still has >= 16 wavefronts per CU
-    const int nt = npts >= (size_t)4096 * 64 ? 4 : npts >= (size_t)4096 * 32 ? 2 : 1;
-    switch (cout / 16) {
-        case 1: launch_conv_fwd<1>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
-        case 2: launch_conv_fwd<2>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
-        case 3: launch_conv_fwd<3>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
-        case 4: launch_conv_fwd<4>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
-        case 6: launch_conv_fwd<6>(nt, npts, u->stream, X, cin, W, kp, bias, Y, n, relu); break;
-        default: return CFM_EINVAL;
-    }
+    if (cout % 16 != 0 || !conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu))
+        return CFM_FAIL(CFM_EINVAL, "unet: no convolution kernel for %d -> %d channels", cin, cout);
     CFM_HIP(hipGetLastError());
     return CFM_OK;
 }

@@ -144,6 +144,21 @@ def test_full_size_2d_properties(gpu):
     assert np.all(np.diff(norms) < 0)
 
 
+@pytest.mark.parametrize("time_block", [0, 1])
+def test_cpu_configuration_1024_10000(gpu, orc, time_block):
+    """The reference's CPU-runnable case at its own size: 2D 1024^2 FP64, sin
+    x faces, Dirichlet, 10000 sweeps, update norm every 100 -- the GPU field
+    equals the oracle's host loop bit for bit (two sweeps per pass and one)."""
+    init = gpu.init_field(1024, 1024)
+    assert np.array_equal(init, orc.init2d(1024, 1024))
+    out, norms, t = gpu.jacobi(init, 10000, bc=gpu.BC_DIRICHLET, norm_every=100, time_block=time_block)
+    ref, rnorms = orc.jacobi(init, 10000, orc.BC_DIRICHLET, 100)
+    assert t["outer_iterations"] == (5000 if time_block == 0 else 10000)
+    assert np.array_equal(out, ref)
+    assert len(norms) == len(rnorms) == 100
+    assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
+
+
 T2_SHAPES = [(3, 5, 6), (9, 13, 512), (17, 23, 511), (40, 37, 100), (12, 70, 257)]
 
 

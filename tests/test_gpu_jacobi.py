@@ -3,9 +3,11 @@ equal the CPU oracle bit for bit (FP64 and FP32 fields; same left-to-right
 sums as Stencil/jacobi2D/src/common.cu:37 and Stencil/jacobi3D/src/common.cu:43),
 for both boundary modes, both execution schemes and every tile shape.  The
 update norms are sums in a different order: relative tolerance 1e-10.
-At the full benchmark sizes (2D 16384^2, 3D 512^3) the checks are the
-size-independent properties: mirror symmetry in x (bitwise), the discrete
-maximum principle, a constant field staying constant, and norms decreasing."""
+At the full benchmark sizes (2D 16384^2, 3D 512^3, 1000 sweeps) and at the
+CPU configuration (2D 1024^2, 10000 sweeps) the field is compared with the
+oracle bit for bit as well, beside the size-independent properties: mirror
+symmetry in x (bitwise), the discrete maximum principle, a constant field
+staying constant, and norms decreasing."""
 import numpy as np
 import pytest
 
@@ -156,6 +158,21 @@ def test_cpu_configuration_1024_10000(gpu, orc, time_block):
     assert t["outer_iterations"] == (5000 if time_block == 0 else 10000)
     assert np.array_equal(out, ref)
     assert len(norms) == len(rnorms) == 100
+    assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+def test_benchmark_configuration_bitexact(gpu, orc, dims):
+    """The GPU benchmark configurations at full size against the oracle:
+    3D 512^3 and 2D 16384^2 FP64, sin x faces, Dirichlet, 1000 sweeps (two
+    per HBM pass), update norm every 100 -- field bit for bit."""
+    shape = (512, 512, 512) if dims == 3 else (16384, 16384)
+    init = gpu.init_field(*shape[::-1])
+    out, norms, t = gpu.jacobi(init, 1000, bc=gpu.BC_DIRICHLET, norm_every=100)
+    assert t["outer_iterations"] == 500
+    ref, rnorms = orc.jacobi(init, 1000, orc.BC_DIRICHLET, 100)
+    del init
+    assert np.array_equal(out, ref)
     assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
 
 

@@ -294,3 +294,27 @@ def test_batch_dia_rejects_unbanded(gpu):
     with pytest.raises(gpu.CfmError) as e:
         gpu.CgBatchPlan(srp, M.indptr, M.indices, M.data, p)
     assert e.value.status == gpu.ENOTSUP
+
+
+def test_benchmark_batch_workload(gpu, orc):
+    """The batched benchmark workload itself (256 systems, edges log-uniform in
+    [32, 96], seed 2024) in one launch: every member reaches 1e-8 by its true
+    residual, and three members (first, middle, last) equal the oracle's solo
+    solve bit for bit."""
+    systems = _systems(gpu, 2024, 256, 32, 96)
+    srp, rp, col, val, b, p = _batch(gpu, systems, tol=1e-8, max_iter=20000)
+    with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:
+        plan.set_rhs(b)
+        plan.set_x0(None)
+        st, iters, stat, hist, t = plan.solve(hist_cap=20001)
+        x = plan.x()
+    assert st == gpu.OK and np.all(stat == gpu.OK)
+    for i, (rp_, col_, val_, f_) in enumerate(systems):
+        r = f_ - sp.csr_matrix((val_, col_, rp_)) @ x[srp[i]:srp[i + 1]]
+        assert np.linalg.norm(r) <= 1.2e-8 * np.linalg.norm(f_), i
+    for i in (0, 128, 255):
+        rp_, col_, val_, f_ = systems[i]
+        ost, ox, oh, oit = orc.cg(orc.Csr(rp_, col_, val_), f_, tol=1e-8, max_iter=20000, variant="pipelined")
+        assert iters[i] == oit
+        assert np.array_equal(x[srp[i]:srp[i + 1]], ox)
+        assert np.array_equal(hist[i, : oit + 1], oh)

@@ -161,13 +161,14 @@ def test_cpu_configuration_1024_10000(gpu, orc, time_block):
     assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
 
 
-@pytest.mark.parametrize("dims", [3, 2])
-def test_benchmark_configuration_bitexact(gpu, orc, dims):
+@pytest.mark.parametrize("dims,prec", [(3, "f64"), (2, "f64"), (3, "f32")])
+def test_benchmark_configuration_bitexact(gpu, orc, dims, prec):
     """The GPU benchmark configurations at full size against the oracle:
-    3D 512^3 and 2D 16384^2 FP64, sin x faces, Dirichlet, 1000 sweeps (two
-    per HBM pass), update norm every 100 -- field bit for bit."""
+    3D 512^3 and 2D 16384^2 FP64 and the 3D FP32 sweep (FP64 norms), sin x
+    faces, Dirichlet, 1000 sweeps (two per HBM pass), update norm every 100
+    -- field bit for bit."""
     shape = (512, 512, 512) if dims == 3 else (16384, 16384)
-    init = gpu.init_field(*shape[::-1])
+    init = gpu.init_field(*shape[::-1], prec=gpu.F64 if prec == "f64" else gpu.F32)
     out, norms, t = gpu.jacobi(init, 1000, bc=gpu.BC_DIRICHLET, norm_every=100)
     assert t["outer_iterations"] == 500
     ref, rnorms = orc.jacobi(init, 1000, orc.BC_DIRICHLET, 100)

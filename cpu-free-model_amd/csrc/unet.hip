@@ -15,7 +15,8 @@
 //
 // Backward data is the same kernel with the weights flipped in space and
 // transposed in channels; backward weights is a split-K MFMA GEMM over point
-// slabs followed by a fixed-order reduction of the slab partials.
+// slabs (one workgroup per tap and slab, point blocks staged transposed in
+// LDS) followed by a fixed-order reduction of the slab partials.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -197,6 +198,91 @@ __global__ void __launch_bounds__(256) conv3_wgrad_kernel(const bf16 *__restrict
     float *out = partial + (size_t)s * Cout * Kp;
 #pragma unroll
     for (int i = 0; i < 4; ++i) out[(size_t)(mt * 16 + 4 * g + i) * Kp + kt * 16 + nl] = acc[i];
+}
+
+// LDS-staged weight gradient for one tap: workgroup (tap, slab) computes
+// partial[slab][co][tap*CIN + ci] for all co < COUT, ci < CIN.  Per block of
+// 64 consecutive points it stages dZ (64 x COUT) and the tap-shifted input
+// (64 x CIN, zero where the neighbour leaves the grid) transposed in LDS,
+// [channel][point], so each lane's 8 consecutive points of one channel -- an
+// MFMA fragment along the point (K) dimension -- are one ds_read_b128.  The
+// (co tile, ci tile) pairs are spread over the four wavefronts.  The tap-26
+// workgroups also zero the padding columns [27*CIN, Kp).
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) conv3_wgrad_lds_kernel(const bf16 *__restrict__ X, const bf16 *__restrict__ dZ,
+                                                              int Kp, float *__restrict__ partial, int n, int npts,
+                                                              int slab) {
+    constexpr int PB = 64;  // points per staged block
+    constexpr int MT = COUT / 16, KT = CIN / 16, T = MT * KT, TW = (T + 3) / 4;
+    static_assert(CIN % 16 == 0 && COUT % 16 == 0, "shape");
+    __shared__ __attribute__((aligned(16))) bf16 dzT[COUT * PB];
+    __shared__ __attribute__((aligned(16))) bf16 xT[CIN * PB];
+    const int tap = blockIdx.x, s = blockIdx.y;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nl = lane & 15, g = lane >> 4;
+    f32x4 acc[TW];
+#pragma unroll
+    for (int i = 0; i < TW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 zero8 = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f,
+                          (bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+    const int pbeg = s * slab, pend = min(npts, pbeg + slab);
+    for (int pb = pbeg; pb < pend; pb += PB) {
+        // stage dZ[pb..pb+63][*] and the tap-shifted X[pb..pb+63][*], transposed
+        for (int v = threadIdx.x; v < PB * (COUT / 8); v += 256) {
+            const int j = v / (COUT / 8), c8 = v % (COUT / 8);
+            const int p = pb + j;
+            const bf16x8 val = p < pend ? *reinterpret_cast<const bf16x8 *>(dZ + (size_t)p * COUT + 8 * c8) : zero8;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dzT[(8 * c8 + i) * PB + j] = val[i];
+        }
+        for (int v = threadIdx.x; v < PB * (CIN / 8); v += 256) {
+            const int j = v / (CIN / 8), c8 = v % (CIN / 8);
+            const int p = pb + j;
+            bf16x8 val = zero8;
+            if (p < pend) {
+                const int x = p % n, y = (p / n) % n, z = (p / (n * n)) % n, b = p / (n * n * n);
+                const int xx = x + dx, yy = y + dy, zz = z + dz;
+                if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
+                    val = *reinterpret_cast<const bf16x8 *>(
+                        X + ((((size_t)b * n + zz) * n + yy) * n + xx) * CIN + 8 * c8);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) xT[(8 * c8 + i) * PB + j] = val[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TW; ++i) {
+            const int t = w + 4 * i;
+            if (t < T) {  // uniform per wavefront
+                const int mt = t / KT, kt = t % KT;
+#pragma unroll
+                for (int h = 0; h < PB / 32; ++h) {
+                    const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&dzT[(mt * 16 + nl) * PB + 32 * h + 8 * g]);
+                    const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&xT[(kt * 16 + nl) * PB + 32 * h + 8 * g]);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // the staged blocks are rewritten next
+    }
+    // D[4g+i][nl] = dW[mt*16 + 4g + i][tap*CIN + kt*16 + nl]
+    float *out = partial + (size_t)s * COUT * Kp;
+#pragma unroll
+    for (int i = 0; i < TW; ++i) {
+        const int t = w + 4 * i;
+        if (t < T) {
+            const int mt = t / KT, kt = t % KT;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(size_t)(mt * 16 + 4 * g + r) * Kp + tap * CIN + kt * 16 + nl] = acc[i][r];
+        }
+    }
+    if (tap == 26)
+        for (int e = threadIdx.x; e < COUT * (Kp - 27 * CIN); e += 256) {
+            const int co = e / (Kp - 27 * CIN), k = 27 * CIN + e % (Kp - 27 * CIN);
+            out[(size_t)co * Kp + k] = 0.f;
+        }
 }
 
 // out[i] = sum_s partial[s][i] (fixed order s = 0, 1, ...), i < count.  Eight
@@ -762,6 +848,29 @@ extern "C" cfm_status cfm_unet_forward(cfm_unet *u, const float *input, float *o
 }
 
 // weight and bias gradients of conv I from its input X and output gradient dZ
+// the LDS-staged weight gradient for the network's (input, output) channel pairs
+// with 16-multiple input channels; the first layer (8 padded input channels)
+// keeps the gather kernel
+static bool wgrad_lds(int cin, int cout, int nslab, hipStream_t st, const bf16 *X, const bf16 *dZ, int kp,
+                      float *partial, int n, int npts, int slab) {
+    const dim3 grid(27, nslab);
+#define CFM_WGRAD_LDS(C, O)                                                                              \
+    if (cin == C && cout == O) {                                                                         \
+        hipLaunchKernelGGL((conv3_wgrad_lds_kernel<C, O>), grid, dim3(256), 0, st, X, dZ, kp, partial, n, \
+                           npts, slab);                                                                  \
+        return true;                                                                                     \
+    }
+    CFM_WGRAD_LDS(16, 16)
+    CFM_WGRAD_LDS(16, 32)
+    CFM_WGRAD_LDS(32, 32)
+    CFM_WGRAD_LDS(32, 64)
+    CFM_WGRAD_LDS(64, 64)
+    CFM_WGRAD_LDS(96, 32)
+    CFM_WGRAD_LDS(48, 16)
+#undef CFM_WGRAD_LDS
+    return false;
+}
+
 static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) {
     const ConvSpec &s = kConv[I];
     const int level = s.level, n = u->n >> level;
@@ -771,8 +880,9 @@ static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) 
     const int ntile_k = u->Kp[I] / 16;
     const int ntile = (s.cout / 16) * ntile_k;
     const size_t count = (size_t)s.cout * u->Kp[I];
-    hipLaunchKernelGGL(conv3_wgrad_kernel, dim3((ntile + 3) / 4, nslab), dim3(256), 0, u->stream, X, s.cin, dZ, s.cout,
-                       u->Kp[I], u->partial, n, (int)npts, (int)slab, ntile_k);
+    if (!wgrad_lds(s.cin, s.cout, nslab, u->stream, X, dZ, u->Kp[I], u->partial, n, (int)npts, (int)slab))
+        hipLaunchKernelGGL(conv3_wgrad_kernel, dim3((ntile + 3) / 4, nslab), dim3(256), 0, u->stream, X, s.cin, dZ,
+                           s.cout, u->Kp[I], u->partial, n, (int)npts, (int)slab, ntile_k);
     CFM_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->partial, nslab, count, u->gk);
     hipLaunchKernelGGL(unpack_wgrad_kernel, blocks_for((size_t)s.cout * s.cin_real * 27), dim3(256), 0, u->stream,

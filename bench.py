@@ -239,7 +239,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              f"(multi-GPU runs are not part of this benchmark)", file=sys.stderr)
         sys.exit(2)
     import cfm
 

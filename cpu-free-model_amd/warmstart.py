@@ -38,7 +38,7 @@ EVAL_SEED = 77003311
 DEFAULT_WEIGHTS = os.path.join(_HERE, "weights", "unet_varcoef64.npz")
 # The network's error has a high-frequency part that inflates ||b - A u0||
 # (the guard's test) while costing CG little; two damped Jacobi sweeps on
-# the guess remove it before the guard judges the guess.
+# the guess remove it before the guard checks the guess.
 SMOOTH_SWEEPS = 2
 SMOOTH_OMEGA = 0.8
 

@@ -238,7 +238,7 @@ def warmstart_figures(nsys):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU runs are not part of this benchmark)", file=sys.stderr)
         sys.exit(2)
     import cfm

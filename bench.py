@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--chunks", type=int, default=0)
-    ap.add_argument("--time-block", type=int, default=0, help="sweeps per HBM pass: 0 auto, 1, 2")
+    ap.add_argument("--time-block", type=int, default=0, help="sweeps per HBM pass (3D): 0 auto, 1, 2")
     ap.add_argument("--block", type=int, default=0, help="threads per workgroup (0 auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")

@@ -1184,8 +1184,208 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
     }
 }
 
+// Three sweeps per pass in 2D: as sweep2d_t2 with one more level.  At step r
+// the wavefront computes level 1 on row r, level 2 on row r-1 and level 3 on
+// row r-2 (stored).  Lanes 0 and 63 carry three level-0 columns outside the
+// segment (e1 adjacent, e2, e3), level 1 on e1 and e2 and level 2 on e1; each
+// chunk recomputes levels 1-2 on the two rows beyond each end.  Every value
+// comes from the single-sweep expression: bit-identical to three sweeps.
+template <typename T, int VX, bool ALIGNED, int LA = 2>
+__device__ __forceinline__ void sweep2d_t3(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
+                                           int tx, int yc, bool want0, bool want1, bool want2, double &n0,
+                                           double &n1, double &n2) {
+    constexpr int SEG = 64 * VX;
+    constexpr int NW = 3 + LA;  // level-0 rows r-1 .. r+1+LA (the last LA in flight)
+    const int lane = threadIdx.x & 63;
+    const int nx = A.nx, ny = A.ny;
+    const int x0 = tx * SEG;
+    const int x = x0 + lane * VX;
+    const int yb = 1 + yc * A.chunk_len;
+    const int ye = min(yb + A.chunk_len, ny - 1);
+    if (yb >= ye) return;  // uniform over the wavefront
+    const T quarter = T(0.25);
+    const bool lane_w = lane == 0, lane_e = lane == 63;
+    const bool edge_lane = lane_w || lane_e;
+    // edge columns: lane 0 -> x0-1, x0-2, x0-3; lane 63 -> x0+SEG, +1, +2
+    int ec[3];
+    bool ein[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ec[i] = lane_w ? x0 - 1 - i : x0 + SEG + i;
+        ein[i] = edge_lane && ec[i] >= 1 && ec[i] <= nx - 2;
+    }
+    bool xin[VX];
+#pragma unroll
+    for (int k = 0; k < VX; ++k) xin[k] = x + k >= 1 && x + k <= nx - 2;
+
+    T a[NW][VX];     // level-0 rows
+    T ae[NW][3];     // their edge columns e1, e2, e3 (edge lanes)
+    T l1[3][VX] = {}, l1e1[3] = {}, l1e2[3] = {};  // level 1, rows r-2, r-1, r
+    T l2[3][VX] = {}, l2e1[3] = {};                // level 2, rows r-3, r-2, r-1
+
+    auto load = [&](int y, T (&v)[VX], T (&e)[3]) {
+        const bool ok = y >= 0 && y < ny;  // uniform
+        const T *pr = src + (size_t)(ok ? y : 0) * nx;
+        load_row<T, VX, ALIGNED>(pr, x, nx, ok, v);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] = (ok && edge_lane && ec[i] >= 0 && ec[i] < nx) ? pr[ec[i]] : T(0);
+    };
+    // one row of a level from the row below/above (S/N) and its lanes' edge value
+    auto row = [&](const T (&cr)[VX], const T (&N)[VX], const T (&S)[VX], T wedge, T eedge, T (&out)[VX],
+                   bool norm, double &acc) {
+        T west = lane_up1(cr[VX - 1]);
+        T east = lane_dn1(cr[0]);
+        if (lane_w) west = wedge;
+        if (lane_e) east = eedge;
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+            const T E = (k < VX - 1) ? cr[k + 1] : east;
+            const T W = (k > 0) ? cr[k - 1] : west;
+            const T v = quarter * (E + W + N[k] + S[k]);
+            out[k] = xin[k] ? v : cr[k];
+            if (norm && xin[k]) {
+                const double d = (double)v - (double)cr[k];
+                acc += d * d;
+            }
+        }
+    };
+
+    const int r0 = yb - 2, r1 = ye + 1;  // level-1 rows
+#pragma unroll
+    for (int i = 0; i < NW - 1; ++i) load(r0 - 1 + i, a[i], ae[i]);
+    for (int r = r0; r <= r1; ++r) {
+        load(r + 1 + LA, a[NW - 1], ae[NW - 1]);  // consumed LA steps later
+        // ---- level 1 on row r (own columns; e1 and e2 for the edge lanes)
+        T n1r[VX], n1e1, n1e2;
+        if (r >= 1 && r <= ny - 2) {  // wave-uniform
+            row(a[1], a[2], a[0], ae[1][0], ae[1][0], n1r, want0 && r >= yb && r < ye, n0);
+            {
+                const T E = lane_w ? a[1][0] : ae[1][1];
+                const T W = lane_w ? ae[1][1] : a[1][VX - 1];
+                const T v = quarter * (E + W + ae[2][0] + ae[0][0]);
+                n1e1 = ein[0] ? v : ae[1][0];
+            }
+            {
+                const T E = lane_w ? ae[1][0] : ae[1][2];
+                const T W = lane_w ? ae[1][2] : ae[1][0];
+                const T v = quarter * (E + W + ae[2][1] + ae[0][1]);
+                n1e2 = ein[1] ? v : ae[1][1];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) n1r[k] = a[1][k];
+            n1e1 = ae[1][0];
+            n1e2 = ae[1][1];
+        }
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+            l1[0][k] = l1[1][k];
+            l1[1][k] = l1[2][k];
+            l1[2][k] = n1r[k];
+        }
+        l1e1[0] = l1e1[1];
+        l1e1[1] = l1e1[2];
+        l1e1[2] = n1e1;
+        l1e2[0] = l1e2[1];
+        l1e2[1] = l1e2[2];
+        l1e2[2] = n1e2;
+        // ---- level 2 on row r-1 (own columns; e1 for the edge lanes)
+        const int q = r - 1;
+        if (q >= yb - 1 && q <= ye) {  // wave-uniform
+            T n2r[VX], n2e1;
+            if (q >= 1 && q <= ny - 2) {
+                row(l1[1], l1[2], l1[0], l1e1[1], l1e1[1], n2r, want1 && q >= yb && q < ye, n1);
+                const T E = lane_w ? l1[1][0] : l1e2[1];
+                const T W = lane_w ? l1e2[1] : l1[1][VX - 1];
+                const T v = quarter * (E + W + l1e1[2] + l1e1[0]);
+                n2e1 = ein[0] ? v : l1e1[1];
+            } else {
+#pragma unroll
+                for (int k = 0; k < VX; ++k) n2r[k] = l1[1][k];
+                n2e1 = l1e1[1];
+            }
+#pragma unroll
+            for (int k = 0; k < VX; ++k) {
+                l2[0][k] = l2[1][k];
+                l2[1][k] = l2[2][k];
+                l2[2][k] = n2r[k];
+            }
+            l2e1[0] = l2e1[1];
+            l2e1[1] = l2e1[2];
+            l2e1[2] = n2e1;
+        }
+        // ---- level 3 on row r-2, stored
+        const int q3 = r - 2;
+        if (q3 >= yb && q3 < ye) {  // wave-uniform; interior rows only
+            T out[VX];
+            row(l2[1], l2[2], l2[0], l2e1[1], l2e1[1], out, want2, n2);
+            store_row_nt<T, VX, ALIGNED>(dst + (size_t)q3 * nx, x, nx, out, xin);
+        }
+        // ---- slide the level-0 window
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) a[i][k] = a[i + 1][k];
+#pragma unroll
+            for (int e = 0; e < 3; ++e) ae[i][e] = ae[i + 1][e];
+        }
+    }
+}
+
+template <typename T, bool ALIGNED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) jacobi2d_t3_kernel(JacobiArgs A) {
+    constexpr int VX = t2d_cols<T>(BLOCK);
+    __shared__ double lds[BLOCK / 64];
+    const int wave_id = xcd_wave_id();
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    const int nunits = A.ntx * A.chunks;
+    unsigned int epoch = 0;
+    int pass = 0;
+    for (int k = 0; k < A.niter; ++pass) {
+        const int steps = min(3, A.niter - k);
+        const int it = A.it0 + k;
+        bool want[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) want[l] = l < steps && A.norm_every > 0 && ((it + l) % A.norm_every) == 0;
+        const T *src = static_cast<const T *>((pass & 1) ? A.buf1 : A.buf0);
+        T *dst = static_cast<T *>((pass & 1) ? A.buf0 : A.buf1);
+        double nrm[3] = {0.0, 0.0, 0.0};
+        for (int u = wave_id; u < nunits; u += nwaves) {
+            const int yc = u / A.ntx;
+            const int tx = u - yc * A.ntx;
+            if (steps == 3)
+                sweep2d_t3<T, VX, ALIGNED>(src, dst, A, tx, yc, want[0], want[1], want[2], nrm[0], nrm[1], nrm[2]);
+            else
+                sweep2d_t2<T, VX, ALIGNED>(src, dst, A, tx, yc, steps == 1, want[0], want[1], nrm[0], nrm[1]);
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            if (want[l]) {  // uniform over the grid
+                const double s = block_sum_waves<BLOCK / 64>(nrm[l], lds);
+                if (threadIdx.x == 0) A.norm_part[(size_t)((it + l) / A.norm_every) * gridDim.x + blockIdx.x] = s;
+            }
+        }
+        k += steps;
+        if (k < A.niter) {
+            if (!grid_barrier(A.sync, epoch)) return;
+        }
+    }
+}
+
 // ------------------------------------------------------------- dispatch ---
 using KernelFn = void (*)(JacobiArgs);
+
+static KernelFn pick_t3d_kernel(bool f64, bool aligned, int block) {
+    if (block == 512)
+        return f64 ? (aligned ? (KernelFn)jacobi2d_t3_kernel<double, true, 512>
+                              : (KernelFn)jacobi2d_t3_kernel<double, false, 512>)
+                   : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 512>
+                              : (KernelFn)jacobi2d_t3_kernel<float, false, 512>);
+    return f64 ? (aligned ? (KernelFn)jacobi2d_t3_kernel<double, true, 1024>
+                          : (KernelFn)jacobi2d_t3_kernel<double, false, 1024>)
+               : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 1024>
+                          : (KernelFn)jacobi2d_t3_kernel<float, false, 1024>);
+}
 
 static KernelFn pick_t2d_kernel(bool f64, bool aligned, int block) {
     if (block == 512)
@@ -1299,7 +1499,8 @@ struct cfm_jacobi_plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     KernelFn kernel = nullptr;
-    bool t2 = false;  // two sweeps per pass (jacobi3d_t2_kernel)
+    bool t2 = false;  // more than one sweep per pass (the two-/three-sweep kernels)
+    int sweeps_per_pass = 1;
     int grid = 0;
     int chunks = 0, chunk_len = 0, ntx = 0, nty = 0, rows = 0, block = 256;
     unsigned long long spin_ticks = 0;
@@ -1313,7 +1514,7 @@ static cfm_status validate(const cfm_jacobi_params *p) {
     if (p->bc != CFM_BC_PERIODIC && p->bc != CFM_BC_DIRICHLET) return CFM_EINVAL;
     if (p->variant < CFM_JACOBI_LAUNCH_PER_ITER || p->variant > CFM_JACOBI_NO_COMPUTE)
         return CFM_EINVAL;
-    if (p->time_block < 0 || p->time_block > 2) return CFM_EINVAL;
+    if (p->time_block < 0 || p->time_block > 3) return CFM_EINVAL;
     // 32-bit plane/row arithmetic inside the kernels
     const double elems = (double)p->nx * p->ny * p->nz;
     if (elems > 9.0e15 || (double)p->nx * p->ny > 2.0e9) return CFM_EINVAL;
@@ -1355,13 +1556,21 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent Dirichlet variant, "
                                      "the default tile shape and, in 3D, nx <= %d", kT2Width);
     }
+    if (p->time_block == 3 && !t2_ok2) {
+        delete pl;
+        return CFM_FAIL(CFM_ENOTSUP, "jacobi: three sweeps per pass needs the persistent 2D Dirichlet variant "
+                                     "and the default tile shape");
+    }
     pl->t2 = t2_ok && p->time_block != 1;
+    // 2D: three sweeps per pass by default (16384^2 FP64 790 vs 617 GLUP/s for two; FP32 1523 vs 1206)
+    pl->sweeps_per_pass = !pl->t2 ? 1 : (p->time_block == 3 || (p->time_block == 0 && !is3d)) ? 3 : 2;
     if (pl->t2 && !is3d) {
         pl->rows = 1;
         // 16384^2: FP64 617 GLUP/s at 1024 threads (2 columns per lane) vs 534 at 512;
         // FP32 1205 at 1024 threads vs 942 at 512
         pl->block = p->block_threads > 0 ? p->block_threads : 1024;
-        pl->kernel = pick_t2d_kernel(f64, aligned, pl->block);
+        pl->kernel = pl->sweeps_per_pass == 3 ? pick_t3d_kernel(f64, aligned, pl->block)
+                                              : pick_t2d_kernel(f64, aligned, pl->block);
     } else if (pl->t2) {
         pl->rows = 1;
         // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428
@@ -1517,14 +1726,16 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
         produced = pl->norm_slots;
     }
     if (norm_count) *norm_count = produced;
-    if (p.variant != CFM_JACOBI_NO_COMPUTE) pl->cur ^= pl->t2 ? (((p.niter + 1) / 2) & 1) : (p.niter & 1);
+    const int spp = pl->sweeps_per_pass;
+    const int passes = (p.niter + spp - 1) / spp;
+    if (p.variant != CFM_JACOBI_NO_COMPUTE) pl->cur ^= passes & 1;
     if (t) {
         t->kernel_ms = ms;
         t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
         t->grid_blocks = pl->grid;
         t->block_threads = pl->block;
         t->iterations = p.niter;
-        t->outer_iterations = p.variant == CFM_JACOBI_NO_COMPUTE ? 0 : pl->t2 ? (p.niter + 1) / 2 : p.niter;
+        t->outer_iterations = p.variant == CFM_JACOBI_NO_COMPUTE ? 0 : passes;
     }
     return CFM_OK;
 }

@@ -78,8 +78,9 @@ typedef struct {
     int chunks;       /* chunks along the streamed axis; 0 = auto */
     int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
     int block_threads;/* 0 = one workgroup per CU (size set by the tile shape), or 256 */
-    int time_block;   /* sweeps per HBM pass of the persistent 3D kernel: 0 = auto (2 for
-                         Dirichlet with nx <= 512 and the default tile shape, else 1), 1, 2 */
+    int time_block;   /* sweeps per HBM pass of the persistent Dirichlet kernels: 0 = auto
+                         (2D: 3; 3D: 2 when nx <= 512; with the default tile shape, else 1),
+                         1, 2, or 3 (2D only) */
 } cfm_jacobi_params;
 
 typedef struct {

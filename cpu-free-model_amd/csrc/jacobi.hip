@@ -1372,8 +1372,208 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t3_kernel(JacobiArgs A) {
     }
 }
 
+// L sweeps per pass in 2D, the general form of sweep2d_t3: at step r level j
+// (1..L) is computed on row r-(j-1); lanes 0 and 63 carry L level-0 columns
+// beyond the segment and level j on L-j of them; each chunk recomputes the
+// rows a level needs beyond its ends.  Levels are compile-time (template
+// recursion), so every window stays in registers.
+template <typename T, int VX, bool ALIGNED, int L, int LA = 2>
+struct Sweep2dN {
+    static constexpr int NW = 3 + LA;
+    const T *__restrict__ src;
+    T *__restrict__ dst;
+    int nx, ny, yb, ye, x, lane;
+    bool lane_w, lane_e, edge_lane;
+    int ec[L];
+    bool ein[L], xin[VX];
+    T a[NW][VX], ae[NW][L];
+    T lv[L - 1][3][VX], lve[L - 1][3][L];
+    const bool *want;
+    double *nrm;
+
+    __device__ __forceinline__ void load(int y, T (&v)[VX], T (&e)[L]) {
+        const bool ok = y >= 0 && y < ny;  // uniform
+        const T *pr = src + (size_t)(ok ? y : 0) * nx;
+        load_row<T, VX, ALIGNED>(pr, x, nx, ok, v);
+#pragma unroll
+        for (int i = 0; i < L; ++i) e[i] = (ok && edge_lane && ec[i] >= 0 && ec[i] < nx) ? pr[ec[i]] : T(0);
+    }
+    // one level-J row from the level below: S/C/N rows and their edge columns
+    template <int J>
+    __device__ __forceinline__ void level_row(const T (&S)[VX], const T (&C)[VX], const T (&N)[VX], const T (&Se)[L],
+                                              const T (&Ce)[L], const T (&Ne)[L], bool rin, bool norm,
+                                              T (&out)[VX], T (&oe)[L]) {
+        constexpr int NE = L - J;  // edge columns computed at this level
+        const T quarter = T(0.25);
+        if (rin) {
+            T west = lane_up1(C[VX - 1]);
+            T east = lane_dn1(C[0]);
+            if (lane_w) west = Ce[0];
+            if (lane_e) east = Ce[0];
+#pragma unroll
+            for (int k = 0; k < VX; ++k) {
+                const T E = (k < VX - 1) ? C[k + 1] : east;
+                const T W = (k > 0) ? C[k - 1] : west;
+                const T v = quarter * (E + W + N[k] + S[k]);
+                out[k] = xin[k] ? v : C[k];
+                if (norm && xin[k]) {
+                    const double d = (double)v - (double)C[k];
+                    nrm[J - 1] += d * d;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NE; ++i) {
+                const T inner = i == 0 ? (lane_w ? C[0] : C[VX - 1]) : Ce[i > 0 ? i - 1 : 0];
+                const T outer = Ce[i + 1];
+                const T E = lane_w ? inner : outer;
+                const T W = lane_w ? outer : inner;
+                const T v = quarter * (E + W + Ne[i] + Se[i]);
+                oe[i] = ein[i] ? v : Ce[i];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) out[k] = C[k];
+#pragma unroll
+            for (int i = 0; i < NE; ++i) oe[i] = Ce[i];
+        }
+    }
+    template <int J>
+    __device__ __forceinline__ void levels(int r) {
+        if constexpr (J <= L) {
+            const int q = r - (J - 1);
+            if (q >= yb - (L - J) && q <= ye - 1 + (L - J)) {  // wave-uniform
+                T out[VX], oe[L];
+                const bool rin = q >= 1 && q <= ny - 2;
+                const bool norm = want[J - 1] && q >= yb && q < ye;
+                if constexpr (J == 1)
+                    level_row<J>(a[0], a[1], a[2], ae[0], ae[1], ae[2], rin, norm, out, oe);
+                else
+                    level_row<J>(lv[J - 2][0], lv[J - 2][1], lv[J - 2][2], lve[J - 2][0], lve[J - 2][1],
+                                 lve[J - 2][2], rin, norm, out, oe);
+                if constexpr (J < L) {
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) {
+                        lv[J - 1][0][k] = lv[J - 1][1][k];
+                        lv[J - 1][1][k] = lv[J - 1][2][k];
+                        lv[J - 1][2][k] = out[k];
+                    }
+#pragma unroll
+                    for (int i = 0; i < L - J; ++i) {
+                        lve[J - 1][0][i] = lve[J - 1][1][i];
+                        lve[J - 1][1][i] = lve[J - 1][2][i];
+                        lve[J - 1][2][i] = oe[i];
+                    }
+                } else if (q >= yb && q < ye) {
+                    store_row_nt<T, VX, ALIGNED>(dst + (size_t)q * nx, x, nx, out, xin);
+                }
+            }
+            levels<J + 1>(r);
+        }
+    }
+    __device__ __forceinline__ void run(int tx, int yc, const JacobiArgs &A) {
+        constexpr int SEG = 64 * VX;
+        lane = threadIdx.x & 63;
+        nx = A.nx;
+        ny = A.ny;
+        const int x0 = tx * SEG;
+        x = x0 + lane * VX;
+        yb = 1 + yc * A.chunk_len;
+        ye = min(yb + A.chunk_len, ny - 1);
+        if (yb >= ye) return;  // uniform over the wavefront
+        lane_w = lane == 0;
+        lane_e = lane == 63;
+        edge_lane = lane_w || lane_e;
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            ec[i] = lane_w ? x0 - 1 - i : x0 + SEG + i;
+            ein[i] = edge_lane && ec[i] >= 1 && ec[i] <= nx - 2;
+        }
+#pragma unroll
+        for (int k = 0; k < VX; ++k) xin[k] = x + k >= 1 && x + k <= nx - 2;
+#pragma unroll
+        for (int j = 0; j < L - 1; ++j)
+#pragma unroll
+            for (int w = 0; w < 3; ++w) {
+#pragma unroll
+                for (int k = 0; k < VX; ++k) lv[j][w][k] = T(0);
+#pragma unroll
+                for (int i = 0; i < L; ++i) lve[j][w][i] = T(0);
+            }
+        const int r0 = yb - L + 1, r1 = ye + L - 2;  // level-1 rows
+#pragma unroll
+        for (int i = 0; i < NW - 1; ++i) load(r0 - 1 + i, a[i], ae[i]);
+        for (int r = r0; r <= r1; ++r) {
+            load(r + 1 + LA, a[NW - 1], ae[NW - 1]);  // consumed LA steps later
+            levels<1>(r);
+#pragma unroll
+            for (int i = 0; i < NW - 1; ++i) {
+#pragma unroll
+                for (int k = 0; k < VX; ++k) a[i][k] = a[i + 1][k];
+#pragma unroll
+                for (int e = 0; e < L; ++e) ae[i][e] = ae[i + 1][e];
+            }
+        }
+    }
+};
+
+template <typename T, bool ALIGNED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) jacobi2d_t4_kernel(JacobiArgs A) {
+    constexpr int VX = t2d_cols<T>(BLOCK);
+    __shared__ double lds[BLOCK / 64];
+    const int wave_id = xcd_wave_id();
+    const int nwaves = gridDim.x * (blockDim.x >> 6);
+    const int nunits = A.ntx * A.chunks;
+    unsigned int epoch = 0;
+    int pass = 0;
+    for (int k = 0; k < A.niter; ++pass) {
+        const int steps = min(4, A.niter - k);
+        const int it = A.it0 + k;
+        bool want[4];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) want[l] = l < steps && A.norm_every > 0 && ((it + l) % A.norm_every) == 0;
+        const T *src = static_cast<const T *>((pass & 1) ? A.buf1 : A.buf0);
+        T *dst = static_cast<T *>((pass & 1) ? A.buf0 : A.buf1);
+        double nrm[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int u = wave_id; u < nunits; u += nwaves) {
+            const int yc = u / A.ntx;
+            const int tx = u - yc * A.ntx;
+            if (steps == 4) {
+                Sweep2dN<T, VX, ALIGNED, 4> sw;
+                sw.src = src;
+                sw.dst = dst;
+                sw.want = want;
+                sw.nrm = nrm;
+                sw.run(tx, yc, A);
+            } else if (steps == 3) {
+                sweep2d_t3<T, VX, ALIGNED>(src, dst, A, tx, yc, want[0], want[1], want[2], nrm[0], nrm[1], nrm[2]);
+            } else {
+                sweep2d_t2<T, VX, ALIGNED>(src, dst, A, tx, yc, steps == 1, want[0], want[1], nrm[0], nrm[1]);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            if (want[l]) {  // uniform over the grid
+                const double s = block_sum_waves<BLOCK / 64>(nrm[l], lds);
+                if (threadIdx.x == 0) A.norm_part[(size_t)((it + l) / A.norm_every) * gridDim.x + blockIdx.x] = s;
+            }
+        }
+        k += steps;
+        if (k < A.niter) {
+            if (!grid_barrier(A.sync, epoch)) return;
+        }
+    }
+}
+
 // ------------------------------------------------------------- dispatch ---
 using KernelFn = void (*)(JacobiArgs);
+
+// four sweeps per pass: 512-thread workgroups only (the 1024-thread build spills)
+static KernelFn pick_t4d_kernel(bool f64, bool aligned) {
+    return f64 ? (aligned ? (KernelFn)jacobi2d_t4_kernel<double, true, 512>
+                          : (KernelFn)jacobi2d_t4_kernel<double, false, 512>)
+               : (aligned ? (KernelFn)jacobi2d_t4_kernel<float, true, 512>
+                          : (KernelFn)jacobi2d_t4_kernel<float, false, 512>);
+}
 
 static KernelFn pick_t3d_kernel(bool f64, bool aligned, int block) {
     if (block == 512)
@@ -1514,7 +1714,7 @@ static cfm_status validate(const cfm_jacobi_params *p) {
     if (p->bc != CFM_BC_PERIODIC && p->bc != CFM_BC_DIRICHLET) return CFM_EINVAL;
     if (p->variant < CFM_JACOBI_LAUNCH_PER_ITER || p->variant > CFM_JACOBI_NO_COMPUTE)
         return CFM_EINVAL;
-    if (p->time_block < 0 || p->time_block > 3) return CFM_EINVAL;
+    if (p->time_block < 0 || p->time_block > 4) return CFM_EINVAL;
     // 32-bit plane/row arithmetic inside the kernels
     const double elems = (double)p->nx * p->ny * p->nz;
     if (elems > 9.0e15 || (double)p->nx * p->ny > 2.0e9) return CFM_EINVAL;
@@ -1556,21 +1756,29 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent Dirichlet variant, "
                                      "the default tile shape and, in 3D, nx <= %d", kT2Width);
     }
-    if (p->time_block == 3 && !t2_ok2) {
+    if (p->time_block >= 3 && !t2_ok2) {
         delete pl;
-        return CFM_FAIL(CFM_ENOTSUP, "jacobi: three sweeps per pass needs the persistent 2D Dirichlet variant "
-                                     "and the default tile shape");
+        return CFM_FAIL(CFM_ENOTSUP, "jacobi: three or four sweeps per pass need the persistent 2D Dirichlet "
+                                     "variant and the default tile shape");
+    }
+    if (p->time_block == 4 && p->block_threads > 0 && p->block_threads != 512) {
+        delete pl;
+        return CFM_FAIL(CFM_ENOTSUP, "jacobi: four sweeps per pass run in 512-thread workgroups");
     }
     pl->t2 = t2_ok && p->time_block != 1;
-    // 2D: three sweeps per pass by default (16384^2 FP64 790 vs 617 GLUP/s for two; FP32 1523 vs 1206)
-    pl->sweeps_per_pass = !pl->t2 ? 1 : (p->time_block == 3 || (p->time_block == 0 && !is3d)) ? 3 : 2;
+    // 2D: four sweeps per pass by default with the default workgroup (16384^2 FP64 976 GLUP/s vs 790
+    // for three and 617 for two; FP32 1612 vs 1523 and 1206); three when 1024 threads are asked for
+    const int auto2d = (p->block_threads <= 0 || p->block_threads == 512) ? 4 : 3;
+    pl->sweeps_per_pass =
+        !pl->t2 ? 1 : p->time_block >= 3 ? p->time_block : (p->time_block == 0 && !is3d) ? auto2d : 2;
     if (pl->t2 && !is3d) {
         pl->rows = 1;
         // 16384^2: FP64 617 GLUP/s at 1024 threads (2 columns per lane) vs 534 at 512;
         // FP32 1205 at 1024 threads vs 942 at 512
-        pl->block = p->block_threads > 0 ? p->block_threads : 1024;
-        pl->kernel = pl->sweeps_per_pass == 3 ? pick_t3d_kernel(f64, aligned, pl->block)
-                                              : pick_t2d_kernel(f64, aligned, pl->block);
+        pl->block = p->block_threads > 0 ? p->block_threads : (pl->sweeps_per_pass == 4 ? 512 : 1024);
+        pl->kernel = pl->sweeps_per_pass == 4   ? pick_t4d_kernel(f64, aligned)
+                     : pl->sweeps_per_pass == 3 ? pick_t3d_kernel(f64, aligned, pl->block)
+                                                : pick_t2d_kernel(f64, aligned, pl->block);
     } else if (pl->t2) {
         pl->rows = 1;
         // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428

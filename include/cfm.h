@@ -79,8 +79,9 @@ typedef struct {
     int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
     int block_threads;/* 0 = one workgroup per CU (size set by the tile shape), or 256 */
     int time_block;   /* sweeps per HBM pass of the persistent Dirichlet kernels: 0 = auto
-                         (2D: 3; 3D: 2 when nx <= 512; with the default tile shape, else 1),
-                         1, 2, or 3 (2D only) */
+                         (2D: 4, or 3 with 1024-thread workgroups; 3D: 2 when nx <= 512; with the
+                         default tile shape, else 1), 1, 2, or 3 and 4 (2D only; 4 in 512-thread
+                         workgroups) */
 } cfm_jacobi_params;
 
 typedef struct {

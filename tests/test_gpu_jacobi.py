@@ -150,12 +150,12 @@ def test_full_size_2d_properties(gpu):
 def test_cpu_configuration_1024_10000(gpu, orc, time_block):
     """The reference's CPU-runnable case at its own size: 2D 1024^2 FP64, sin
     x faces, Dirichlet, 10000 sweeps, update norm every 100 -- the GPU field
-    equals the oracle's host loop bit for bit (three sweeps per pass and one)."""
+    equals the oracle's host loop bit for bit (four sweeps per pass and one)."""
     init = gpu.init_field(1024, 1024)
     assert np.array_equal(init, orc.init2d(1024, 1024))
     out, norms, t = gpu.jacobi(init, 10000, bc=gpu.BC_DIRICHLET, norm_every=100, time_block=time_block)
     ref, rnorms = orc.jacobi(init, 10000, orc.BC_DIRICHLET, 100)
-    assert t["outer_iterations"] == (3334 if time_block == 0 else 10000)  # auto: three sweeps per pass in 2D
+    assert t["outer_iterations"] == (2500 if time_block == 0 else 10000)  # auto: four sweeps per pass in 2D
     assert np.array_equal(out, ref)
     assert len(norms) == len(rnorms) == 100
     assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
@@ -165,12 +165,12 @@ def test_cpu_configuration_1024_10000(gpu, orc, time_block):
 def test_benchmark_configuration_bitexact(gpu, orc, dims, prec):
     """The GPU benchmark configurations at full size against the oracle:
     3D 512^3 and 2D 16384^2 FP64 and the 3D FP32 sweep (FP64 norms), sin x
-    faces, Dirichlet, 1000 sweeps (two per HBM pass in 3D, three in 2D),
+    faces, Dirichlet, 1000 sweeps (two per HBM pass in 3D, four in 2D),
     update norm every 100 -- field bit for bit."""
     shape = (512, 512, 512) if dims == 3 else (16384, 16384)
     init = gpu.init_field(*shape[::-1], prec=gpu.F64 if prec == "f64" else gpu.F32)
     out, norms, t = gpu.jacobi(init, 1000, bc=gpu.BC_DIRICHLET, norm_every=100)
-    assert t["outer_iterations"] == (500 if dims == 3 else 334)  # two sweeps per pass in 3D, three in 2D
+    assert t["outer_iterations"] == (500 if dims == 3 else 250)  # two sweeps per pass in 3D, four in 2D
     ref, rnorms = orc.jacobi(init, 1000, orc.BC_DIRICHLET, 100)
     del init
     assert np.array_equal(out, ref)
@@ -252,8 +252,28 @@ def test_three_sweeps_per_pass_norms_and_units_2d(gpu, orc, norm_every, chunks, 
     _check(gpu, orc, init, 11, 1, norm_every=norm_every, time_block=3, chunks=chunks, grid_blocks=grid)
 
 
-def test_three_sweeps_per_pass_rejected_in_3d(gpu):
-    p = gpu.jacobi_params(64, 16, 8, niter=3, bc=1, time_block=3)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", T2_SHAPES_2D)
+@pytest.mark.parametrize("niter", [1, 2, 3, 4, 5, 7, 9])
+def test_four_sweeps_per_pass_bitexact_2d(gpu, orc, dtype, shape, niter):
+    """Four sweeps per pass in 2D (the general level recursion; remainders run
+    the three-, two- or one-sweep pass) equal single sweeps bit for bit."""
+    init = _rand(shape, dtype, 41 + niter)
+    t = _check(gpu, orc, init, niter, 1, norm_every=2, time_block=4)
+    assert t["outer_iterations"] == (niter + 3) // 4
+
+
+@pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (9, 3), (400, 2)])
+def test_four_sweeps_per_pass_units_2d(gpu, orc, chunks, grid):
+    init = _rand((421, 700), np.float64, 8)
+    _check(gpu, orc, init, 13, 1, norm_every=3, time_block=4, chunks=chunks, grid_blocks=grid)
+
+
+@pytest.mark.parametrize("kw", [dict(nz=8, time_block=3), dict(nz=8, time_block=4), dict(nz=1, time_block=4,
+                                                                                          block_threads=1024)])
+def test_three_and_four_sweeps_per_pass_rejections(gpu, kw):
+    nz = kw.pop("nz")
+    p = gpu.jacobi_params(64, 16, nz, niter=4, bc=1, **kw)
     with pytest.raises(gpu.CfmError) as e:
         gpu.JacobiPlan(p)
     assert e.value.status == gpu.ENOTSUP

@@ -6,8 +6,8 @@
 // "Baseline *" run one launch per sweep (the reference's single_gpu loop),
 // "Design 1/2" run the single persistent cooperative kernel one sweep per
 // pass, "PERKS" the persistent kernel that keeps the intermediate sweeps on
-// chip (two sweeps per HBM pass in 3D, three in 2D, where the shape allows
-// it; -tblock 1|2|3 to force), and the "(No computation)" versions run the
+// chip (two sweeps per HBM pass in 3D, four in 2D, where the shape allows
+// it; -tblock 1|2|3|4 to force), and the "(No computation)" versions run the
 // persistent kernel's barriers only.
 //
 // -compare runs the launch-per-sweep scheme first (the reference's

@@ -85,6 +85,18 @@ struct CgArgs {
         CHUNK_END                                                                           \
     }
 
+// Store that writes through the XCD's L2 (scope bit sc1) instead of leaving a
+// dirty line there.  Standard CG writes s, x, r and p in phases separated by
+// device barriers, and the next phase reads them from other XCDs; with these
+// stores a 128^3 Poisson iteration takes 76.4 instead of 82.1 us (Pipelined CG,
+// one barrier per iteration, measured no gain and keeps plain stores).  The
+// compiler does not count this store in its own memory wait counters; the
+// waits it inserts can only get longer for it, and the release at every
+// device barrier waits for all outstanding stores.
+__device__ __forceinline__ void store_wt(double *p, double v) {
+    asm volatile("global_store_dwordx2 %0, %1, off sc1" ::"v"(p), "v"(v) : "memory");
+}
+
 // =============================================================== phases ===
 // Standard CG.  v[0]=r, v[1]=p, v[2]=s.  part: [0]=gamma, [1]=delta, [2]=bb.
 struct StdPhases {
@@ -130,7 +142,7 @@ struct StdPhases {
     }
         CFM_FOR_MY_ROWS({
             const double si = sell_row<double, double>(A.A64, p, slice, lane);
-            s[row] = si;
+            store_wt(&s[row], si);
             ad = fma(p[row], si, ad);
         })
 #undef CHUNK_END
@@ -148,9 +160,9 @@ struct StdPhases {
         ag = 0.0;                                                                           \
     }
         CFM_FOR_MY_ROWS({
-            A.x[row] = fma(alpha, p[row], A.x[row]);
+            store_wt(&A.x[row], fma(alpha, p[row], A.x[row]));
             const double ri = fma(-alpha, s[row], r[row]);
-            r[row] = ri;
+            store_wt(&r[row], ri);
             ag = fma(ri, ri, ag);
         })
 #undef CHUNK_END
@@ -161,7 +173,7 @@ struct StdPhases {
         const double *r = A.v[0];
         double *p = A.v[1];
 #define CHUNK_END
-        CFM_FOR_MY_ROWS({ p[row] = fma(beta, p[row], r[row]); })
+        CFM_FOR_MY_ROWS({ store_wt(&p[row], fma(beta, p[row], r[row])); })
 #undef CHUNK_END
     }
 };

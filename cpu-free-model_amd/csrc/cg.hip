@@ -86,14 +86,15 @@ struct CgArgs {
     }
 
 // Store that writes through the XCD's L2 (scope bit sc1) instead of leaving a
-// dirty line there.  Standard CG writes s, x, r and p in phases separated by
-// device barriers, and the next phase reads them from other XCDs; with these
-// stores a 128^3 Poisson iteration takes 76.4 instead of 82.1 us; the FP32
-// inner iteration of the refinement (three barriers too) 59.7 instead of
-// 60.6 us.  Pipelined CG, one barrier per iteration, measured no gain and
-// keeps plain stores.  The compiler does not count these stores in its own
-// memory wait counters; the waits it inserts can only get longer for them,
-// and the release at every device barrier waits for all outstanding stores.
+// dirty line there for the next device barrier's release to write back.
+// Measured on the 128^3 Poisson system: Standard CG (s, x, r, p; three
+// barriers per iteration) 82.1 -> 76.4 us per iteration; the refinement's FP32
+// inner iteration (s, d, r, p) 60.6 -> 59.7 us; Pipelined CG (one barrier per
+// iteration) gains only for w, the vector the next SpMV gathers (71.6 -> 70.7
+// us), and runs slower with its other vectors written through.  The compiler
+// does not count these stores in its own memory wait counters; the waits it
+// inserts can only get longer for them, and the release at every device
+// barrier waits for all outstanding stores.
 __device__ __forceinline__ void store_wt(double *p, double v) {
     asm volatile("global_store_dwordx2 %0, %1, off sc1" ::"v"(p), "v"(v) : "memory");
 }
@@ -265,7 +266,7 @@ struct PipePhases {
             s[row] = si;
             p[row] = pi;
             r[row] = ri;
-            wn[row] = wn_i;
+            store_wt(&wn[row], wn_i);  // gathered by other workgroups next iteration
             ad = fma(ri, ri, ad);
             ag = fma(ri, wn_i, ag);
         })

@@ -1014,6 +1014,12 @@ constexpr int t2d_cols(int block) {
     return (block == 512 ? 2 : 1) * 16 / (int)sizeof(T);
 }
 
+// One 16-byte vector per lane leaves as a non-temporal store.  With two or
+// more per lane each store instruction covers every other 16 bytes of the row,
+// and non-temporal pieces of one 64-byte block are not always merged before
+// they leave the cache; for FP64 plain stores let the L2 merge them (16384^2,
+// four sweeps per pass: counted writes 1.051 -> 1.000 x the field, 975 -> 983
+// GLUP/s).  FP32 rows measured faster non-temporal (1614 vs 1606 GLUP/s).
 template <typename T, int VX, bool ALIGNED>
 __device__ __forceinline__ void store_row_nt(T *drow, int x, int nx, const T (&v)[VX], const bool (&xin)[VX]) {
     constexpr int E = 16 / sizeof(T);
@@ -1023,7 +1029,11 @@ __device__ __forceinline__ void store_row_nt(T *drow, int x, int nx, const T (&v
             typename Vec16<T>::type t;
 #pragma unroll
             for (int e = 0; e < E; ++e) t[e] = v[q * E + e];
-            store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E), t);
+            auto *dp = reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E);
+            if constexpr (sizeof(T) == 8 && VX / E > 1)
+                *dp = t;
+            else
+                store16(dp, t);
         }
     } else {
 #pragma unroll

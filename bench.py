@@ -58,13 +58,37 @@ def parse():
     return ap.parse_args()
 
 
+def host_description():
+    """lscpu fields of the host whose cores time the CPU baseline."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    want = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for ln in out.splitlines():
+        k, _, v = ln.partition(":")
+        if k.strip() in want:
+            v = v.strip()
+            info[want[k.strip()]] = int(v) if v.isdigit() else v
+    return info
+
+
 def cpu_baseline(n, target_s):
-    """Oracle (C, OpenMP) 3D 7-point FP64 sweeps on the host cores, on a
-    bounded number of sweeps of the same n^3 field.  The fixed cost of a
-    call (array copies, first touch) is removed by differencing two runs."""
+    """The oracle (C, OpenMP; the CPU restatement of the reference's host loops)
+    on the host cores, as a reported baseline:
+      * value: a bounded sample of the benchmark workload itself, 3D 7-point
+        FP64 sweeps of the n^3 field, all host threads (the fixed cost of a call
+        -- array copies, first touch -- removed by differencing two runs);
+      * config1_2d_1024: the reference's CPU-runnable case in full, 2D 1024^2
+        FP64, 10000 sweeps, update norm every 100, on all threads and on 1;
+      * cg_128_standard: Standard CG to 1e-8 on the 128^3 Poisson system."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    import numpy as np
+    cores = oracle.lib().orc_get_threads()
     init = oracle.init3d(n, n, n)
 
     def timed(k):
@@ -78,9 +102,34 @@ def cpu_baseline(n, target_s):
     tn = timed(sweeps)
     dt = max(tn - t1, 1e-9)
     lups = (n - 2) ** 3 * (sweeps - 1)
-    return {"value": round(lups / dt / 1e9, 3), "unit": "GLUP/s", "cores": cores, "kind": "port",
-            "sample": f"{sweeps} vs 1 sweeps of the {n}^3 FP64 field (C oracle, OpenMP {cores} threads): "
-                      f"{dt:.1f} s for {sweeps - 1} sweeps"}
+    del init
+    res = {"value": round(lups / dt / 1e9, 3), "unit": "GLUP/s", "cores": cores, "kind": "port",
+           "sample": f"{sweeps} vs 1 sweeps of the {n}^3 FP64 field (C oracle, OpenMP {cores} threads): "
+                     f"{dt:.1f} s for {sweeps - 1} sweeps",
+           "host": host_description()}
+    # config 1: 2D 1024^2 FP64, 10000 sweeps, norms every 100 (the oracle's host loop, timed whole)
+    init2 = oracle.init2d(1024, 1024)
+    c1 = {}
+    for threads in (cores, 1):
+        prev = oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        _, norms = oracle.jacobi(init2, 10000, oracle.BC_DIRICHLET, 100)
+        el = time.perf_counter() - t0
+        oracle.set_threads(prev)
+        lup = 1022 * 1022 * 10000
+        c1[f"threads_{threads}"] = {"seconds": round(el, 3), "glups": round(lup / el / 1e9, 3),
+                                    "gbs_at_16B": round(16 * lup / el / 1e9, 1), "norms": len(norms)}
+    res["config1_2d_1024"] = c1
+    # CG: the oracle's Standard CG (CG/src/common.cu:399-538 restated) on the 128^3 Poisson system
+    A = oracle.poisson3d(128, 128, 128)
+    b = oracle.uniform(0, 0, A.n)
+    t0 = time.perf_counter()
+    st, x, hist, it = oracle.cg(A, b, tol=1e-8, max_iter=10000, variant="standard")
+    el = time.perf_counter() - t0
+    res["cg_128_standard"] = {"seconds_to_1e-8": round(el, 3), "iterations": int(it),
+                              "ms_per_iter": round(1000 * el / max(it, 1), 3), "threads": cores,
+                              "rel_residual": float(f"{np.sqrt(hist[it] / hist[0]):.3e}")}
+    return res
 
 
 def cg_figures(cfm):
@@ -145,23 +194,26 @@ def cg_profile_run(cfm):
     return out
 
 
-def jacobi_f32_figures(cfm, n, niter):
+def jacobi_f32_figures(cfm, shape, niter):
     """FP32 sweep (FP64 norms) against the FP64 sweep, whose field equals the
-    oracle's bit for bit: rate and the field's error after niter sweeps."""
+    oracle's bit for bit: rate and the field's error after niter sweeps, on one
+    field shape ((nz, ny, nx) 3D or (ny, nx) 2D), Dirichlet, default kernels."""
     import numpy as np
+    nx, ny = shape[-1], shape[-2]
+    nz = shape[0] if len(shape) == 3 else 1
+    interior = (nx - 2) * (ny - 2) * (nz - 2 if nz > 1 else 1)
     res = {}
     fields = {}
     for prec, name in ((cfm.F64, "f64"), (cfm.F32, "f32")):
-        init = cfm.init_field(n, n, n, cfm.INIT_SIN, prec)
-        p = cfm.jacobi_params(n, n, n, niter=niter, prec=prec, bc=cfm.BC_DIRICHLET, variant=cfm.PERSISTENT)
+        p = cfm.jacobi_params(nx, ny, nz, niter=niter, prec=prec, bc=cfm.BC_DIRICHLET, variant=cfm.PERSISTENT)
         with cfm.JacobiPlan(p) as plan:
-            plan.load(init)
-            del init
+            plan.load(cfm.init_field(nx, ny, nz, cfm.INIT_SIN, prec))
             plan.run()
-            plan.load(cfm.init_field(n, n, n, cfm.INIT_SIN, prec))
+            plan.load(cfm.init_field(nx, ny, nz, cfm.INIT_SIN, prec))
             _, t = plan.run()
             fields[name] = plan.download()
-        res[name] = {"glups": round((n - 2) ** 3 * niter / t.kernel_ms / 1e6, 1), "ms": round(t.kernel_ms, 2)}
+        res[name] = {"glups": round(interior * niter / t.kernel_ms / 1e6, 1), "ms": round(t.kernel_ms, 2),
+                     "passes": t.outer_iterations}
     d = fields["f32"].astype(np.float64) - fields["f64"]
     res["f32"]["max_abs_err_vs_f64"] = float(f"{np.abs(d).max():.3e}")
     res["f32"]["rel_l2_err_vs_f64"] = float(f"{np.linalg.norm(d) / np.linalg.norm(fields['f64']):.3e}")
@@ -279,6 +331,11 @@ def main():
     grid = t.grid_blocks
     block = t.block_threads
     plan.close()
+    if t.cooperative != 1:
+        # an ordinary launch does not guarantee co-residency: not a valid headline
+        print("bench.py: the persistent kernel went out as an ordinary launch (another process held the "
+              "cooperative-launch grant); no figure is reported for this run", file=sys.stderr)
+        sys.exit(3)
 
     interior = (n - 2) ** 3
     lups = interior * niter * args.steps
@@ -310,7 +367,7 @@ def main():
         "data": "synthetic (sin*sin Dirichlet x faces, zero interior)",
         "config": {"workload": "jacobi3d_7pt_persistent", "nx": n, "ny": n, "nz": n, "sweeps_per_step": niter,
                    "bc": "dirichlet", "grid_blocks": grid, "block_threads": block, "device": dev["name"],
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU", "launch": "cooperative"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": ("cfm::jacobi3d_t2_kernel (FP64, 7-point, two sweeps per HBM pass)" if passes < niter
@@ -323,7 +380,8 @@ def main():
     if not args.no_cg:
         line["cg_128"] = cg_figures(cfm)
     if not args.no_cg:
-        line["jacobi_precision"] = jacobi_f32_figures(cfm, n, niter)
+        line["jacobi_precision"] = {"3d_512": jacobi_f32_figures(cfm, (n, n, n), niter),
+                                    "2d_16384": jacobi_f32_figures(cfm, (16384, 16384), niter)}
     if not args.no_batch:
         line["cg_batch"] = cg_batch_figures(cfm, args.batch_systems)
     if not args.no_warmstart:

@@ -40,7 +40,7 @@ EXPORTS = (
     "cfm_unet_param_count", "cfm_unet_create", "cfm_unet_destroy", "cfm_unet_init_params", "cfm_unet_set_params",
     "cfm_unet_get_params", "cfm_unet_forward", "cfm_unet_gradient", "cfm_unet_train_step",
     "cfm_cg_batch_x0_rejected", "cfm_cg_batch_load_features", "cfm_cg_batch_warm_start", "cfm_gen_varcoef3d_coef",
-    "cfm_cg_batch_smooth_x0",
+    "cfm_cg_batch_smooth_x0", "cfm_gen_tridiag", "cfm_mm_read", "cfm_host_free",
 )
 
 
@@ -60,7 +60,8 @@ class JacobiParams(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("grid_blocks", ctypes.c_int), ("block_threads", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int)]
+                ("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int),
+                ("cooperative", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -162,6 +163,12 @@ def _load():
     L.cfm_cg_batch_warm_start.argtypes = [_vp, _vp, ctypes.c_double, ctypes.POINTER(Timing)]
     L.cfm_gen_varcoef3d_coef.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
     L.cfm_cg_batch_smooth_x0.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Timing)]
+    L.cfm_gen_tridiag.argtypes = [ctypes.c_int, ctypes.c_uint, _vp, _vp, _vp]
+    L.cfm_gen_tridiag.restype = ctypes.c_int64
+    L.cfm_mm_read.argtypes = [ctypes.c_char_p, _ip, _ip, _ip, ctypes.POINTER(_ip), ctypes.POINTER(_ip),
+                              ctypes.POINTER(_dp)]
+    L.cfm_host_free.argtypes = [_vp]
+    L.cfm_host_free.restype = None
     return L
 
 
@@ -209,6 +216,81 @@ def bandwidth_probe(nbytes=1 << 30, reps=5):
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------- device plumbing --
+# Device memory for callers that hand the library device pointers (the
+# borrowed-CSR path, the no-allocation entry point): allocated through the same
+# HIP runtime libcfm is linked against, never through another one.
+
+_hip_rt = None
+
+
+def hip_runtime():
+    global _hip_rt
+    if _hip_rt is None:
+        rt = ctypes.CDLL("libamdhip64.so.7")  # the soname libcfm resolved: the same loaded runtime
+        rt.hipMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
+        rt.hipFree.argtypes = [_vp]
+        rt.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        rt.hipDeviceSynchronize.argtypes = []
+        for fn in (rt.hipMalloc, rt.hipFree, rt.hipMemcpy, rt.hipDeviceSynchronize):
+            fn.restype = ctypes.c_int
+        _hip_rt = rt
+    return _hip_rt
+
+
+_H2D, _D2H = 1, 2
+
+
+class DeviceBuffer:
+    """A plain hipMalloc allocation of nbytes (ptr: its device address)."""
+
+    def __init__(self, nbytes):
+        rt = hip_runtime()
+        self.nbytes = int(nbytes)
+        self.ptr = _vp()
+        if rt.hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise MemoryError(f"hipMalloc({self.nbytes}) failed")
+
+    @classmethod
+    def of(cls, array):
+        array = np.ascontiguousarray(array)
+        buf = cls(array.nbytes)
+        buf.upload(array)
+        return buf
+
+    def upload(self, array):
+        array = np.ascontiguousarray(array)
+        if array.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if hip_runtime().hipMemcpy(self.ptr, _ptr(array), array.nbytes, _H2D) != 0:
+            raise RuntimeError("hipMemcpy host -> device failed")
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("request larger than the buffer")
+        if hip_runtime().hipMemcpy(_ptr(out), self.ptr, out.nbytes, _D2H) != 0:
+            raise RuntimeError("hipMemcpy device -> host failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip_runtime().hipFree(self.ptr)
+            self.ptr = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ Jacobi --
@@ -312,6 +394,36 @@ def poisson3d(nx, ny, nz):
     return rowptr, col, val
 
 
+def tridiag(n, seed=1):
+    """The reference's default CG matrix (genTridiag): returns (rowptr, col, val), 0-based."""
+    nnz = lib.cfm_gen_tridiag(n, seed, None, None, None)
+    if nnz < 0:
+        raise ValueError("n must be >= 2")
+    rowptr = np.empty(n + 1, dtype=np.int32)
+    col = np.empty(nnz, dtype=np.int32)
+    val = np.empty(nnz, dtype=np.float64)
+    if lib.cfm_gen_tridiag(n, seed, _ptr(rowptr), _ptr(col), _ptr(val)) != nnz:
+        raise RuntimeError("cfm_gen_tridiag failed")
+    return rowptr, col, val
+
+
+def mm_read(path):
+    """The reference's Matrix Market reader: returns (rowptr, col, val, index_base).
+    Raises CfmError with the reader's message for files the reference refuses."""
+    n, nnz, base = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    rp, cp, vp = _ip(), _ip(), _dp()
+    _check(lib.cfm_mm_read(os.fsencode(path), ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(base),
+                           ctypes.byref(rp), ctypes.byref(cp), ctypes.byref(vp)), "cfm_mm_read")
+    try:
+        rowptr = np.ctypeslib.as_array(rp, shape=(n.value + 1,)).copy()
+        col = np.ctypeslib.as_array(cp, shape=(max(nnz.value, 1),))[: nnz.value].copy()
+        val = np.ctypeslib.as_array(vp, shape=(max(nnz.value, 1),))[: nnz.value].copy()
+    finally:
+        for q in (rp, cp, vp):
+            lib.cfm_host_free(ctypes.cast(q, _vp))
+    return rowptr, col, val, base.value
+
+
 def uniform(seed, stream_id, count):
     out = np.empty(count, dtype=np.float64)
     _check(lib.cfm_gen_uniform(seed, stream_id, out.ctypes.data_as(_dp), count), "cfm_gen_uniform")
@@ -326,18 +438,30 @@ def cg_params(method=CG_STANDARD, mode=CG_PERSISTENT, tol=1e-8, max_iter=10000, 
 
 
 class CgPlan:
-    """Device-resident CSR (stored as SELL-64) plus CG work vectors."""
+    """Device-resident CSR (stored as SELL-64 or DIA) plus CG work vectors.
 
-    def __init__(self, rowptr, col, val, params, index_base=0):
-        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
-        self.col = np.ascontiguousarray(col, dtype=np.int32)
-        self.val = np.ascontiguousarray(val, dtype=np.float64)
-        self.n = self.rowptr.size - 1
+    rowptr/col/val are host arrays, or DeviceBuffer objects holding a CSR the
+    caller already uploaded (the reference's calling convention: main copies
+    the CSR to the device and hands init() the device pointers,
+    CG/src/main.cu:188-197); the library reads a borrowed CSR once."""
+
+    def __init__(self, rowptr, col, val, params, index_base=0, n=None, nnz=None):
         self.params = params
         h = _vp()
-        _check(lib.cfm_cg_plan_create(self.n, self.col.size, _ptr(self.rowptr), _ptr(self.col), _ptr(self.val),
-                                      index_base, 0, ctypes.byref(params), ctypes.byref(h)),
-               "cfm_cg_plan_create")
+        if isinstance(rowptr, DeviceBuffer):
+            if n is None or nnz is None:
+                raise ValueError("a device CSR needs n and nnz")
+            self.n = int(n)
+            st = lib.cfm_cg_plan_create(self.n, int(nnz), rowptr.ptr, col.ptr, val.ptr, index_base, 1,
+                                        ctypes.byref(params), ctypes.byref(h))
+        else:
+            self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+            self.col = np.ascontiguousarray(col, dtype=np.int32)
+            self.val = np.ascontiguousarray(val, dtype=np.float64)
+            self.n = self.rowptr.size - 1
+            st = lib.cfm_cg_plan_create(self.n, self.col.size, _ptr(self.rowptr), _ptr(self.col), _ptr(self.val),
+                                        index_base, 0, ctypes.byref(params), ctypes.byref(h))
+        _check(st, "cfm_cg_plan_create")
         self._h = h
 
     def set_rhs(self, b):

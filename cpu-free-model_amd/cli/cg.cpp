@@ -3,8 +3,10 @@
 //
 // Matrix: -nx/-ny/-nz build the 3D 7-point Poisson system (default 128^3);
 // -tridiag N builds the reference's random SPD tridiagonal (CG/src/common.cu:46-76,
-// glibc rand(), default seed); -matrix_path reads a Matrix Market file
-// (coordinate, real/integer/pattern, general or symmetric).
+// glibc rand(), default seed); -matrix_path reads a Matrix Market file by the
+// reference's rules (coordinate, real or integer, general / symmetric /
+// skew-symmetric; 0- or 1-based indices detected as CG/src/main.cu:180-186).
+// -json adds one JSON line per run (iterations, residual, times, launch kind).
 // Right-hand side: -rhs uniform (SplitMix64 seed -seed, default) or -rhs ones
 // (the reference's b = 1).  -tol 0 runs exactly -niter iterations as the
 // reference does; otherwise the solve stops at ||r|| <= tol ||b||.
@@ -26,73 +28,39 @@ namespace {
 
 struct Csr {
     int n = 0;
+    int base = 0;  // 0 or 1, as rowptr[0] (the reference's zero_indexed flag)
     std::vector<int> rowptr, col;
     std::vector<double> val;
 };
 
-void gen_tridiag(Csr &A, int n) {
-    // CG/src/common.cu:46-76, same rand() call order
-    const int nnz = (n - 2) * 3 + 4;
+// The reference's default matrix (genTridiag, CG/src/common.cu:46-76) and its
+// Matrix Market path (loadMMSparseMatrix, CG/src/mmio_wrapper.cpp:170-369)
+// live in the library: cfm_gen_tridiag and cfm_mm_read.
+bool gen_tridiag(Csr &A, int n) {
+    const int64_t nnz = cfm_gen_tridiag(n, 1u, nullptr, nullptr, nullptr);
+    if (nnz < 0) return false;
     A.n = n;
-    A.rowptr.assign(n + 1, 0);
-    A.col.assign(nnz, 0);
-    A.val.assign(nnz, 0.0);
-    int *I = A.rowptr.data(), *J = A.col.data();
-    double *val = A.val.data();
-    I[0] = 0, J[0] = 0, J[1] = 1;
-    val[0] = (double)rand() / RAND_MAX + 10.0f;
-    val[1] = (double)rand() / RAND_MAX;
-    for (int i = 1; i < n; i++) {
-        if (i > 1)
-            I[i] = I[i - 1] + 3;
-        else
-            I[1] = 2;
-        const int start = (i - 1) * 3 + 2;
-        J[start] = i - 1;
-        J[start + 1] = i;
-        if (i < n - 1) J[start + 2] = i + 1;
-        val[start] = val[start - 1];
-        val[start + 1] = (double)rand() / RAND_MAX + 10.0f;
-        if (i < n - 1) val[start + 2] = (double)rand() / RAND_MAX;
-    }
-    I[n] = nnz;
+    A.base = 0;
+    A.rowptr.resize((size_t)n + 1);
+    A.col.resize((size_t)nnz);
+    A.val.resize((size_t)nnz);
+    return cfm_gen_tridiag(n, 1u, A.rowptr.data(), A.col.data(), A.val.data()) == nnz;
 }
 
-bool read_matrix_market(const std::string &path, Csr &A) {
-    std::ifstream in(path);
-    if (!in) return false;
-    std::string line;
-    std::getline(in, line);
-    if (line.rfind("%%MatrixMarket", 0) != 0) return false;
-    const bool pattern = line.find("pattern") != std::string::npos;
-    const bool symmetric = line.find("symmetric") != std::string::npos;
-    if (line.find("coordinate") == std::string::npos || line.find("complex") != std::string::npos) return false;
-    while (std::getline(in, line))
-        if (!line.empty() && line[0] != '%') break;
-    long rows = 0, cols = 0, entries = 0;
-    std::istringstream hdr(line);
-    hdr >> rows >> cols >> entries;
-    if (rows <= 0 || rows != cols || rows > 0x7ffffff0) return false;
-    std::vector<std::map<int, double>> r(rows);
-    for (long k = 0; k < entries; ++k) {
-        long i, j;
-        double v = 1.0;
-        if (!(in >> i >> j)) return false;
-        if (!pattern && !(in >> v)) return false;
-        if (i < 1 || j < 1 || i > rows || j > rows) return false;
-        r[i - 1][(int)(j - 1)] += v;
-        if (symmetric && i != j) r[j - 1][(int)(i - 1)] += v;
-    }
-    A.n = (int)rows;
-    A.rowptr.assign(rows + 1, 0);
-    for (long i = 0; i < rows; ++i) {
-        for (auto &kv : r[i]) {
-            A.col.push_back(kv.first);
-            A.val.push_back(kv.second);
-        }
-        A.rowptr[i + 1] = (int)A.col.size();
-    }
-    return true;
+int read_matrix_market(const std::string &path, Csr &A) {
+    int n = 0, nnz = 0, base = 0, *rp = nullptr, *cp = nullptr;
+    double *vp = nullptr;
+    const int st = cfm_mm_read(path.c_str(), &n, &nnz, &base, &rp, &cp, &vp);
+    if (st != CFM_OK) return st;
+    A.n = n;
+    A.base = base;
+    A.rowptr.assign(rp, rp + n + 1);
+    A.col.assign(cp, cp + nnz);
+    A.val.assign(vp, vp + nnz);
+    cfm_host_free(rp);
+    cfm_host_free(cp);
+    cfm_host_free(vp);
+    return CFM_OK;
 }
 
 }  // namespace
@@ -131,6 +99,7 @@ int main(int argc, char *argv[]) {
     const int quirk = get_argval<int>(b, e, "-quirk", 1);
     const int grid = get_argval<int>(b, e, "-grid", 0);
     const int block = get_argval<int>(b, e, "-block", 0);
+    const bool json = get_arg(b, e, "-json");
 
     std::vector<int> versions;
     {
@@ -140,12 +109,10 @@ int main(int argc, char *argv[]) {
     }
     Csr A;
     if (!matrix_path.empty()) {
-        if (!read_matrix_market(matrix_path, A)) {
-            std::fprintf(stderr, "ERROR: cannot read Matrix Market file %s\n", matrix_path.c_str());
-            return 1;
-        }
+        const int st = read_matrix_market(matrix_path, A);
+        if (st != CFM_OK) return fail("reading the Matrix Market file", st);
     } else if (tridiag > 0) {
-        gen_tridiag(A, tridiag);
+        if (!gen_tridiag(A, tridiag)) return fail("cfm_gen_tridiag", CFM_EINVAL);
     } else {
         const int64_t nnz = cfm_gen_poisson3d(nx, ny, nz, nullptr, nullptr, nullptr);
         if (nnz < 0) return fail("cfm_gen_poisson3d", CFM_EINVAL);
@@ -155,7 +122,7 @@ int main(int argc, char *argv[]) {
         A.val.resize(nnz);
         cfm_gen_poisson3d(nx, ny, nz, A.rowptr.data(), A.col.data(), A.val.data());
     }
-    const int nnz = A.rowptr[A.n];
+    const int nnz = A.rowptr[A.n] - A.base;
     std::vector<double> bvec(A.n, 1.0);
     if (rhs != "ones") cfm_gen_uniform(seed, 0, bvec.data(), A.n);
 
@@ -170,8 +137,8 @@ int main(int argc, char *argv[]) {
         p.block_threads = block;
         x.assign(A.n, 0.0);
         hist.assign(iter_max + 2, 0.0);
-        return cfm_cg_solve(A.n, nnz, A.rowptr.data(), A.col.data(), A.val.data(), 0, bvec.data(), x.data(), &p,
-                            hist.data(), (int)hist.size(), &iters, &t);
+        return cfm_cg_solve(A.n, nnz, A.rowptr.data(), A.col.data(), A.val.data(), A.base, bvec.data(), x.data(),
+                            &p, hist.data(), (int)hist.size(), &iters, &t);
     };
 
     std::vector<double> x_ref;
@@ -220,6 +187,12 @@ int main(int argc, char *argv[]) {
                             "status: %s\n",
                             iters, t.outer_iterations, std::sqrt(hist[nh] / bb), t.kernel_ms, t.grid_blocks,
                             t.block_threads, cfm_status_str(st));
+                if (json)
+                    std::printf("{\"version\": \"%s\", \"iterations\": %d, \"outer_iterations\": %d, "
+                                "\"rel_residual\": %.6e, \"kernel_ms\": %.4f, \"wall_ms\": %.4f, \"grid\": %d, "
+                                "\"block\": %d, \"status\": \"%s\", \"cooperative\": %d, \"n\": %d, \"nnz\": %d}\n",
+                                names[v], iters, t.outer_iterations, std::sqrt(hist[nh] / bb), t.kernel_ms,
+                                t.wall_ms, t.grid_blocks, t.block_threads, cfm_status_str(st), t.cooperative, A.n, nnz);
             } else {
                 return 1;
             }

@@ -147,9 +147,10 @@ inline int run_jacobi(int argc, char **argv, int dims) {
     if (json) {
         const double lups = (double)(p.nx - 2) * (p.ny - 2) * (dims == 2 ? 1 : p.nz - 2) * p.niter;
         std::printf("{\"version\": \"%s\", \"kernel_ms\": %.4f, \"glups\": %.3f, \"gbs\": %.1f, \"grid\": %d, "
-                    "\"block\": %d}\n",
+                    "\"block\": %d, \"passes\": %d, \"cooperative\": %d}\n",
                     names[selection], t.kernel_ms, lups / (t.kernel_ms * 1e6),
-                    lups * 2 * esz / (t.kernel_ms * 1e6), t.grid_blocks, t.block_threads);
+                    lups * 2 * esz / (t.kernel_ms * 1e6), t.grid_blocks, t.block_threads, t.outer_iterations,
+                    t.cooperative);
     }
     return 0;
 }

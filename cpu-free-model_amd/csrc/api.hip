@@ -116,7 +116,10 @@ template <typename T>
 static void init_field(int nx, int ny, int nz, int kind, T *a) {
     const size_t plane = (size_t)nx * ny;
     std::memset(a, 0, sizeof(T) * plane * (nz >= 3 ? nz : 1));
-    const double pi = 2.0 * std::asin(1.0);
+    // pi has the element type, as the reference's `const real PI{static_cast<real>(2.0 *
+    // std::asin(1.0))}` (Stencil/jacobi2D/include/common.h:24): float-rounded for the
+    // FP32 field, then widened to double in the sin argument
+    const double pi = (double)(T)(2.0 * std::asin(1.0));
     if (nz < 3) {
         // Stencil/jacobi2D/src/common.cu:15-26 (offset 0)
         for (int iy = 0; iy < ny; ++iy) {

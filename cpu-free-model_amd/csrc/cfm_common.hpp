@@ -117,13 +117,31 @@ __device__ __forceinline__ bool grid_barrier(GridSync *s, unsigned int &epoch) {
 // the co-resident maximum of an otherwise idle device, so all workgroups still
 // start together; if one never became resident (another process filling the
 // CUs) the barrier watchdog ends the launch and it reports CFM_ETIMEOUT.
-inline hipError_t launch_persistent(const void *fn, int grid, int block, void **args, hipStream_t stream) {
+// *cooperative (may be null) receives 1 for a cooperative launch, 0 for the
+// ordinary-launch fallback; callers report it in cfm_timing.cooperative.
+inline hipError_t launch_persistent(const void *fn, int grid, int block, void **args, hipStream_t stream,
+                                    int *cooperative = nullptr) {
     hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+    int coop = 1;
     if (e == hipErrorIllegalState) {
         (void)hipGetLastError();
         e = hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+        coop = 0;
     }
+    if (cooperative) *cooperative = coop;
     return e;
+}
+
+// Status of a persistent launch whose barrier watchdog fired: the message
+// says whether co-residency was guaranteed (cooperative launch) or only
+// expected (ordinary-launch fallback), so a timeout can be told apart from a
+// workgroup that never became resident.
+inline cfm_status barrier_timeout(const char *what, int cooperative) {
+    set_error("%s: a device-wide barrier exceeded its spin limit (%s)", what,
+              cooperative ? "cooperative launch"
+                          : "ordinary launch: another process held the cooperative-launch grant, so "
+                            "co-residency of the workgroups was not guaranteed");
+    return CFM_ETIMEOUT;
 }
 
 // --------------------------------------------------------- reductions ----

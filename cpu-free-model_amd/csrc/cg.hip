@@ -85,22 +85,25 @@ struct CgArgs {
         CHUNK_END                                                                           \
     }
 
-// Store that writes through the XCD's L2 (scope bit sc1) instead of leaving a
-// dirty line there for the next device barrier's release to write back.
-// Measured on the 128^3 Poisson system: Standard CG (s, x, r, p; three
-// barriers per iteration) 82.1 -> 76.4 us per iteration; the refinement's FP32
-// inner iteration (s, d, r, p) 60.6 -> 59.7 us; Pipelined CG (one barrier per
+// Store that writes through the XCD's L2 (a relaxed agent-scope atomic store:
+// a vector store with scope bit sc1 on gfx950) instead of leaving a dirty
+// line there for the next device barrier's release to write back.  Measured
+// on the 128^3 Poisson system: Standard CG (s, x, r, p; three barriers per
+// iteration) 82.1 -> 76.4 us per iteration; the refinement's FP32 inner
+// iteration (s, d, r, p) 60.6 -> 59.7 us; Pipelined CG (one barrier per
 // iteration) gains only for w, the vector the next SpMV gathers (71.6 -> 70.7
-// us), and runs slower with its other vectors written through.  The compiler
-// does not count these stores in its own memory wait counters; the waits it
-// inserts can only get longer for them, and the release at every device
-// barrier waits for all outstanding stores.
+// us), and runs slower with its other vectors written through.  Being a
+// compiler-visible store, it is counted in the compiler's own wait counters.
+// Ordering against other workgroups comes only from the device barrier (a
+// workgroup barrier, then one thread's agent-scope release, which the HIP
+// memory model makes cumulative over the workgroup's earlier stores; an
+// agent-scope acquire on exit); the relaxed atomic adds no ordering of its own.
 __device__ __forceinline__ void store_wt(double *p, double v) {
-    asm volatile("global_store_dwordx2 %0, %1, off sc1" ::"v"(p), "v"(v) : "memory");
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void store_wt(float *p, float v) {
-    asm volatile("global_store_dword %0, %1, off sc1" ::"v"(p), "v"(v) : "memory");
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // =============================================================== phases ===
@@ -737,6 +740,12 @@ extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, cons
     if (p->mode != CFM_CG_PERSISTENT && p->mode != CFM_CG_DISCRETE) return CFM_EINVAL;
     if (p->mode == CFM_CG_DISCRETE && p->method == CFM_CG_IR_F32) return CFM_ENOTSUP;
     if (p->max_iter < 0 || n > 0x7fffffff - 2048) return CFM_EINVAL;
+    if (p->matrix_format < 0 || p->matrix_format > 2) return CFM_EINVAL;
+    // a host CSR is checked before any device work; a borrowed device CSR right after its copy
+    if (!csr_on_device) {
+        const cfm_status vst = check_csr(n, nnz, rowptr, col, index_base, n, "cg");
+        if (vst != CFM_OK) return vst;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_EDEVICE;
 
@@ -757,8 +766,10 @@ extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, cons
         cp = h_col.data();
         vp = h_val.data();
     }
-    if (rp[0] != index_base || rp[n] - index_base != nnz) return CFM_EINVAL;
-    if (p->matrix_format < 0 || p->matrix_format > 2) return CFM_EINVAL;
+    if (csr_on_device) {  // every builder below (DIA, SELL-64) reads the CSR only after this check
+        const cfm_status vst = check_csr(n, nnz, rp, cp, index_base, n, "cg");
+        if (vst != CFM_OK) return vst;
+    }
     std::vector<int> sp, sc, doffs;
     std::vector<double> sv;
     const int npad = (n + kChunk - 1) / kChunk * kChunk;
@@ -1017,7 +1028,7 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
     CgArgs A = make_args(pl);
     A.hist_cap = cap;
     if (!hist) A.hist = nullptr;
-    int status = CFM_OK, k = 0, outer = 0;
+    int status = CFM_OK, k = 0, outer = 0, coop = -1;
     float ms = 0.f;
     if (pl->p.mode == CFM_CG_PERSISTENT) {
         hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
@@ -1025,13 +1036,13 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
         CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
         void *args[] = {&A};
         CFM_HIP(launch_persistent((const void *)pl->K.persistent(pl->p.method), pl->grid, kThreads * pl->sub, args,
-                                  pl->stream));
+                                  pl->stream, &coop));
         CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
         CFM_HIP(hipEventSynchronize(pl->ev1));
         CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
         GridSync hs{};
         CFM_HIP(hipMemcpy(&hs, pl->sync, sizeof(hs), hipMemcpyDeviceToHost));
-        if (hs.abort) return CFM_ETIMEOUT;
+        if (hs.abort) return barrier_timeout("cg", coop);
         int info[4] = {0, 0, 0, 0};
         CFM_HIP(hipMemcpy(info, pl->info, sizeof(int) * 4, hipMemcpyDeviceToHost));
         status = info[0];
@@ -1060,6 +1071,7 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
         t->block_threads = kThreads * pl->sub;
         t->iterations = k;
         t->outer_iterations = outer;
+        t->cooperative = coop;
     }
     return (cfm_status)status;
 }

@@ -600,6 +600,19 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
     if (p->max_iter < 0 || p->method != CFM_CG_PIPELINED || p->mode != CFM_CG_PERSISTENT)
         return CFM_FAIL(CFM_EINVAL, "batched CG: Pipelined persistent method and max_iter >= 0 required");
     if (sys_row_ptr[0] != 0) return CFM_FAIL(CFM_EINVAL, "batched CG: sys_row_ptr[0] must be 0");
+    for (int sy = 0; sy < nsys; ++sy)
+        if (sys_row_ptr[sy + 1] <= sys_row_ptr[sy] || sys_row_ptr[sy + 1] - sys_row_ptr[sy] > 0x3fffffff)
+            return CFM_FAIL(CFM_EINVAL, "batched CG: system %d has %lld rows", sy,
+                            (long long)(sys_row_ptr[sy + 1] - sys_row_ptr[sy]));
+    // rows non-decreasing before any builder reads col/val (columns are checked per system below)
+    if (sys_row_ptr[nsys] > 0x7fffffffLL)
+        return CFM_FAIL(CFM_EINVAL, "batched CG: %lld rows exceed 2^31", (long long)sys_row_ptr[nsys]);
+    {
+        const long long rows = sys_row_ptr[nsys];
+        const cfm_status vst = check_csr(rows, (long long)rowptr[rows] - rowptr[0], rowptr, nullptr, 0, rows,
+                                         "batched CG");
+        if (vst != CFM_OK) return vst;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_FAIL(CFM_EDEVICE, "no GPU");
     auto *pl = new (std::nothrow) cfm_cg_batch_plan();
@@ -761,7 +774,8 @@ extern "C" cfm_status cfm_cg_batch_get_x(cfm_cg_batch_plan *pl, double *x) {
     return CFM_OK;
 }
 
-static cfm_status batch_run(cfm_cg_batch_plan *pl, int max_iter, double tol, int hist_cap, float *ms) {
+static cfm_status batch_run(cfm_cg_batch_plan *pl, int max_iter, double tol, int hist_cap, float *ms,
+                            int *coop) {
     BatchArgs A{};
     A.nsys = pl->nsys;
     A.total_chunks = pl->total_chunks;
@@ -803,13 +817,13 @@ static cfm_status batch_run(cfm_cg_batch_plan *pl, int max_iter, double tol, int
     CFM_HIP(hipGetLastError());
     CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
     void *args[] = {&A};
-    CFM_HIP(launch_persistent((const void *)batch_kernel(pl->sub), pl->grid, 256 * pl->sub, args, pl->stream));
+    CFM_HIP(launch_persistent((const void *)batch_kernel(pl->sub), pl->grid, 256 * pl->sub, args, pl->stream, coop));
     CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
     CFM_HIP(hipEventSynchronize(pl->ev1));
     CFM_HIP(hipEventElapsedTime(ms, pl->ev0, pl->ev1));
     GridSync hs{};
     CFM_HIP(hipMemcpy(&hs, pl->sync, sizeof(hs), hipMemcpyDeviceToHost));
-    if (hs.abort) return CFM_ETIMEOUT;
+    if (hs.abort) return barrier_timeout("cg batch", *coop);
     return CFM_OK;
 }
 
@@ -826,7 +840,8 @@ extern "C" cfm_status cfm_cg_batch_solve(cfm_cg_batch_plan *pl, int *iters, int 
         pl->hist_cap = cap;
     }
     float ms = 0.f;
-    cfm_status st = batch_run(pl, pl->p.max_iter, pl->p.tol, cap, &ms);
+    int coop = -1;
+    cfm_status st = batch_run(pl, pl->p.max_iter, pl->p.tol, cap, &ms, &coop);
     if (st != CFM_OK) return st;
     std::vector<int> h_it(pl->nsys), h_st(pl->nsys);
     CFM_HIP(hipMemcpy(h_it.data(), pl->iters, sizeof(int) * pl->nsys, hipMemcpyDeviceToHost));
@@ -853,6 +868,7 @@ extern "C" cfm_status cfm_cg_batch_solve(cfm_cg_batch_plan *pl, int *iters, int 
         t->block_threads = 256 * pl->sub;
         t->iterations = (int)sum_it;
         t->outer_iterations = 0;
+        t->cooperative = coop;
     }
     return overall;
 }
@@ -898,7 +914,8 @@ extern "C" cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *pl, int trial_iters, 
         for (int m = 1; m <= per_cu; ++m) {
             if (batch_configure(pl, sub, m * pl->dev.cus) != CFM_OK) continue;
             float ms = 0.f;
-            cfm_status st = batch_run(pl, trial_iters, 0.0, 0, &ms);
+            int coop = -1;
+            cfm_status st = batch_run(pl, trial_iters, 0.0, 0, &ms, &coop);
             if (st != CFM_OK) return st;
             if (ms < best) {
                 best = ms;
@@ -970,6 +987,7 @@ extern "C" cfm_status cfm_cg_batch_warm_start(cfm_cg_batch_plan *pl, cfm_unet *n
         t->block_threads = 0;
         t->iterations = (pl->nsys + nv.batch - 1) / nv.batch;
         t->outer_iterations = 0;
+        t->cooperative = -1;
     }
     return CFM_OK;
 }
@@ -1005,6 +1023,7 @@ extern "C" cfm_status cfm_cg_batch_smooth_x0(cfm_cg_batch_plan *pl, int sweeps, 
         t->block_threads = 256;
         t->iterations = sweeps;
         t->outer_iterations = 0;
+        t->cooperative = -1;
     }
     return CFM_OK;
 }

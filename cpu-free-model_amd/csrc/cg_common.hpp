@@ -92,4 +92,28 @@ __device__ __forceinline__ double reduce_partials(const double *part, int nchunk
 
 __device__ __forceinline__ bool is_finite(double v) { return v - v == 0.0; }
 
+// Host check of a CSR before any storage builder reads it: rowptr[0] = base,
+// rows non-decreasing, rowptr[n] = nnz + base, and (when col is given) every
+// column in [base, ncols + base).  Records the first violation for
+// cfm_last_error() and returns CFM_EINVAL; the builders then index col/val
+// only inside [0, nnz).
+inline cfm_status check_csr(long long n, long long nnz, const int *rowptr, const int *col, int base,
+                            long long ncols, const char *what) {
+    if (rowptr[0] != base)
+        return CFM_FAIL(CFM_EINVAL, "%s: rowptr[0] = %d, expected the index base %d", what, rowptr[0], base);
+    for (long long r = 0; r < n; ++r)
+        if (rowptr[r + 1] < rowptr[r])
+            return CFM_FAIL(CFM_EINVAL, "%s: rowptr decreases at row %lld (%d > %d)", what, r, rowptr[r],
+                            rowptr[r + 1]);
+    if ((long long)rowptr[n] - base != nnz)
+        return CFM_FAIL(CFM_EINVAL, "%s: rowptr[n] - base = %lld, nnz = %lld", what, (long long)rowptr[n] - base,
+                        nnz);
+    if (col)
+        for (long long j = 0; j < nnz; ++j)
+            if ((long long)col[j] < base || (long long)col[j] >= ncols + base)
+                return CFM_FAIL(CFM_EINVAL, "%s: column %d of entry %lld is outside [%d, %lld)", what, col[j], j,
+                                base, ncols + base);
+    return CFM_OK;
+}
+
 }  // namespace cfm

@@ -1820,7 +1820,8 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
         delete pl;
         return CFM_EDEVICE;
     }
-    const int max_grid = per_cu * pl->dev.cus;
+    // the barrier keeps one arrival word per workgroup (kMaxGrid of them)
+    const int max_grid = std::min(per_cu * pl->dev.cus, kMaxGrid);
     pl->grid = p->grid_blocks > 0 ? std::min(p->grid_blocks, max_grid) : max_grid;
     // work units: one per wavefront (per-wave kernels) or per workgroup (3D tile kernel)
     const bool wg_tile = is3d && pl->block != 256;
@@ -1909,6 +1910,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
     hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
     CFM_HIP(hipGetLastError());
     CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
+    int coop = -1;
     if (p.niter > 0) {
         if (p.variant == CFM_JACOBI_LAUNCH_PER_ITER) {
             for (int k = 0; k < p.niter; ++k) {
@@ -1921,7 +1923,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
             A.niter = p.niter;
             A.it0 = 0;
             void *args[] = {&A};
-            CFM_HIP(launch_persistent((const void *)pl->kernel, pl->grid, pl->block, args, pl->stream));
+            CFM_HIP(launch_persistent((const void *)pl->kernel, pl->grid, pl->block, args, pl->stream, &coop));
         }
     }
     CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
@@ -1930,7 +1932,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
     CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
     GridSync hs{};
     CFM_HIP(hipMemcpy(&hs, pl->sync, sizeof(hs), hipMemcpyDeviceToHost));
-    if (hs.abort) return CFM_ETIMEOUT;
+    if (hs.abort) return barrier_timeout("jacobi", coop);
     int produced = 0;
     if (pl->norm_slots > 0 && p.variant != CFM_JACOBI_NO_COMPUTE) {
         std::vector<double> part((size_t)pl->norm_slots * pl->grid);
@@ -1954,6 +1956,7 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
         t->block_threads = pl->block;
         t->iterations = p.niter;
         t->outer_iterations = p.variant == CFM_JACOBI_NO_COMPUTE ? 0 : passes;
+        t->cooperative = coop;
     }
     return CFM_OK;
 }

@@ -91,6 +91,10 @@ typedef struct {
     int block_threads;  /* threads per workgroup */
     int iterations;     /* sweeps (Jacobi) or CG iterations (summed inner for IR) */
     int outer_iterations; /* Jacobi: passes over the field in HBM; IR: refinements */
+    int cooperative;    /* persistent kernel launch: 1 = cooperative (co-residency guaranteed),
+                           0 = ordinary launch because another process held ROCm's one-process
+                           cooperative grant (co-residency expected, not guaranteed; the barrier
+                           watchdog bounds every wait); -1 = no persistent launch */
 } cfm_timing;
 
 typedef struct cfm_jacobi_plan cfm_jacobi_plan;
@@ -242,6 +246,30 @@ cfm_status cfm_gen_varcoef3d_coef(uint64_t seed, int system_id, int lmin, int lm
                                   double *kcell);
 /* SplitMix64 stream (seed, stream_id) -> count doubles in [0,1). */
 cfm_status cfm_gen_uniform(uint64_t seed, uint64_t stream_id, double *out, int64_t count);
+
+/* ------------------------------------------- the reference's CG inputs --- */
+/* The reference's default CG matrix, genTridiag (CG/src/common.cu:46-76; n =
+ * 20,971,520 in CG/src/main.cu:118-129): a random symmetric tridiagonal
+ * matrix, diagonal rand()/RAND_MAX + 10, off-diagonals rand()/RAND_MAX, drawn
+ * in the reference's call order from a private copy of the C library's rand()
+ * stream seeded with `seed` (1 = an unseeded program, the reference's case;
+ * the caller's rand() state is untouched).  0-based CSR, nnz = 3n - 2.
+ * Returns nnz, or -1 for n < 2; NULL arrays query nnz. */
+int64_t cfm_gen_tridiag(int n, unsigned int seed, int *rowptr, int *col, double *val);
+/* The reference's Matrix Market path: loadMMSparseMatrix
+ * (CG/src/mmio_wrapper.cpp:170-369) over mm_read_mtx_crd (CG/src/mmio.c) and
+ * main's index-base test (CG/src/main.cu:180-186).  Coordinate files with real
+ * or integer data, general / symmetric / skew-symmetric storage (expanded to
+ * both triangles); entries sorted by (row, column); pattern, complex, array,
+ * repeated entries and both-based indices are refused (CFM_EINVAL with a
+ * message), as the reference refuses them; non-square matrices are refused
+ * too.  *index_base = 0 if an index 0 occurs, 1 if an index equal to the
+ * dimension occurs, else 0; rowptr starts at the base and columns keep it,
+ * ready for cfm_cg_plan_create(..., index_base, ...).  The three arrays are
+ * allocated by the library; release each with cfm_host_free. */
+cfm_status cfm_mm_read(const char *path, int *n, int *nnz, int *index_base, int **rowptr, int **col,
+                       double **val);
+void cfm_host_free(void *ptr);
 
 /* ---------------------------------------------------- learned warm start --- */
 /* No counterpart in the reference: a 3-level 3D U-Net that maps (f, log k)

@@ -20,10 +20,15 @@
  * Stencil/jacobi2D/src/common.cu:215-249 (the reference never evaluates it).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
+
+/* Threads of the oracle's OpenMP loops (the CPU baseline times 1 thread and all cores). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 void orc_jacobi_init2d_f64(double *a, int nx, int ny, int kind) {
     /* Stencil/jacobi2D/src/common.cu:15-26 with offset 0: both x faces get
@@ -54,6 +59,40 @@ void orc_jacobi_init3d_f64(double *a, int nx, int ny, int nz, int kind) {
         const double sz = sin(2.0 * pi * iz / (nz - 1));
         for (int iy = 0; iy < ny; ++iy) {
             const double v = sin(2.0 * pi * iy / (ny - 1)) * sz;
+            a[iz * plane + (size_t)iy * nx + 0] = v;
+            a[iz * plane + (size_t)iy * nx + nx - 1] = v;
+        }
+    }
+}
+
+/* FP32 fields: the reference's pi is `real`, i.e. float-rounded for float fields
+ * (Stencil/jacobi2D/include/common.h:24, Stencil/jacobi3D/include/common.h:24), and
+ * widened back to double in the sin argument; values are then rounded to float. */
+void orc_jacobi_init2d_f32(float *a, int nx, int ny, int kind) {
+    (void)kind;
+    const double pi = (double)(float)(2.0 * asin(1.0));
+    memset(a, 0, sizeof(float) * (size_t)nx * ny);
+    for (int iy = 0; iy < ny; ++iy) {
+        const float y0 = (float)sin(2.0 * pi * iy / (ny - 1));
+        a[(size_t)iy * nx + 0] = y0;
+        a[(size_t)iy * nx + nx - 1] = y0;
+    }
+}
+
+void orc_jacobi_init3d_f32(float *a, int nx, int ny, int nz, int kind) {
+    const size_t plane = (size_t)nx * ny;
+    memset(a, 0, sizeof(float) * plane * nz);
+    if (kind == ORC_INIT_REF_QUIRK) {
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy)
+                a[iz * plane + (size_t)iy * nx] = (float)(((double)iz - (double)iy) - (double)(nx - 1));
+        return;
+    }
+    const double pi = (double)(float)(2.0 * asin(1.0));
+    for (int iz = 0; iz < nz; ++iz) {
+        const double sz = sin(2.0 * pi * iz / (nz - 1));
+        for (int iy = 0; iy < ny; ++iy) {
+            const float v = (float)(sin(2.0 * pi * iy / (ny - 1)) * sz);
             a[iz * plane + (size_t)iy * nx + 0] = v;
             a[iz * plane + (size_t)iy * nx + nx - 1] = v;
         }

@@ -53,6 +53,10 @@ enum {
 
 void orc_jacobi_init2d_f64(double *a, int nx, int ny, int kind);
 void orc_jacobi_init3d_f64(double *a, int nx, int ny, int nz, int kind);
+void orc_jacobi_init2d_f32(float *a, int nx, int ny, int kind);
+void orc_set_threads(int n);
+int orc_get_threads(void);
+void orc_jacobi_init3d_f32(float *a, int nx, int ny, int nz, int kind);
 
 /* Jacobi solvers.  `init` holds the starting field (both working arrays start
  * from it, as the reference initialises a and a_new alike).  `out` receives

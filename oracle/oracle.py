@@ -47,6 +47,12 @@ def lib():
         c_int, c_double, c_u64, c_i64 = ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_int64
         L.orc_jacobi_init2d_f64.argtypes = [_dp, c_int, c_int, c_int]
         L.orc_jacobi_init3d_f64.argtypes = [_dp, c_int, c_int, c_int, c_int]
+        L.orc_jacobi_init2d_f32.argtypes = [_fp, c_int, c_int, c_int]
+        L.orc_set_threads.argtypes = [c_int]
+        L.orc_set_threads.restype = None
+        L.orc_get_threads.argtypes = []
+        L.orc_get_threads.restype = c_int
+        L.orc_jacobi_init3d_f32.argtypes = [_fp, c_int, c_int, c_int, c_int]
         for nm, p in (("orc_jacobi2d_f64", _dp), ("orc_jacobi2d_f32", _fp)):
             getattr(L, nm).argtypes = [p, p, c_int, c_int, c_int, c_int, c_int, _dp, c_int]
             getattr(L, nm).restype = c_int
@@ -81,17 +87,31 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (returns the previous count)."""
+    prev = lib().orc_get_threads()
+    lib().orc_set_threads(int(n))
+    return prev
+
+
 # ------------------------------------------------------------------ Jacobi --
 
-def init2d(nx, ny, kind=INIT_SIN):
-    a = np.empty((ny, nx), dtype=np.float64)
-    lib().orc_jacobi_init2d_f64(_ptr(a, _dp), nx, ny, kind)
+def init2d(nx, ny, kind=INIT_SIN, dtype=np.float64):
+    """Initial 2D field; float32 uses the float-rounded pi of the reference's `real` PI."""
+    a = np.empty((ny, nx), dtype=dtype)
+    if a.dtype == np.float32:
+        lib().orc_jacobi_init2d_f32(_ptr(a, _fp), nx, ny, kind)
+    else:
+        lib().orc_jacobi_init2d_f64(_ptr(a, _dp), nx, ny, kind)
     return a
 
 
-def init3d(nx, ny, nz, kind=INIT_SIN):
-    a = np.empty((nz, ny, nx), dtype=np.float64)
-    lib().orc_jacobi_init3d_f64(_ptr(a, _dp), nx, ny, nz, kind)
+def init3d(nx, ny, nz, kind=INIT_SIN, dtype=np.float64):
+    a = np.empty((nz, ny, nx), dtype=dtype)
+    if a.dtype == np.float32:
+        lib().orc_jacobi_init3d_f32(_ptr(a, _fp), nx, ny, nz, kind)
+    else:
+        lib().orc_jacobi_init3d_f64(_ptr(a, _dp), nx, ny, nz, kind)
     return a
 
 

@@ -2,6 +2,7 @@
 declared symbol, validates arguments, and its host-side generators agree with
 the oracle bit for bit."""
 import ctypes
+import math
 import os
 import re
 
@@ -49,7 +50,31 @@ def test_init_fields_match_oracle(cfm, orc):
     assert np.array_equal(cfm.init_field(9, 8, 7), orc.init3d(9, 8, 7))
     assert np.array_equal(cfm.init_field(9, 8, 7, cfm.INIT_REF_QUIRK), orc.init3d(9, 8, 7, orc.INIT_REF_QUIRK))
     f32 = cfm.init_field(9, 8, 7, prec=cfm.F32)
-    assert f32.dtype == np.float32 and np.array_equal(f32, orc.init3d(9, 8, 7).astype(np.float32))
+    assert f32.dtype == np.float32 and np.array_equal(f32, orc.init3d(9, 8, 7, dtype=np.float32))
+    assert np.array_equal(cfm.init_field(19, 13, prec=cfm.F32), orc.init2d(19, 13, dtype=np.float32))
+    assert np.array_equal(cfm.init_field(9, 8, 7, cfm.INIT_REF_QUIRK, prec=cfm.F32),
+                          orc.init3d(9, 8, 7, orc.INIT_REF_QUIRK, dtype=np.float32))
+
+
+def test_f32_init_uses_the_float_rounded_pi(cfm, orc):
+    # The reference's PI is `real` (float): static_cast<real>(2*asin(1)), widened to double
+    # inside sin(2.0*pi*iy/(ny-1)) (Stencil/jacobi2D/include/common.h:24, src/common.cu:20).
+    # Expected values from CPython's math.sin (the C library's sin), independent of both.
+    ny = 1024
+    pi_f = float(np.float32(2.0 * math.asin(1.0)))
+    want = np.array([math.sin(2.0 * pi_f * iy / (ny - 1)) for iy in range(ny)]).astype(np.float32)
+    got = cfm.init_field(8, ny, prec=cfm.F32)
+    assert np.array_equal(got[:, 0], want) and np.array_equal(got[:, -1], want)
+    assert np.array_equal(orc.init2d(8, ny, dtype=np.float32)[:, 0], want)
+    # the double-precision pi gives different float values on most rows: the test can tell them apart
+    f64pi = cfm.init_field(8, ny)[:, 0].astype(np.float32)
+    assert np.count_nonzero(f64pi != want) > ny // 4
+    # 3D faces: sin(2 pi iy/(ny-1)) * sin(2 pi iz/(nz-1)) in double with the same pi, then float
+    nz, ny3 = 11, 9
+    want3 = np.array([[math.sin(2.0 * pi_f * iy / (ny3 - 1)) * math.sin(2.0 * pi_f * iz / (nz - 1))
+                       for iy in range(ny3)] for iz in range(nz)]).astype(np.float32)
+    g3 = cfm.init_field(6, ny3, nz, prec=cfm.F32)
+    assert np.array_equal(g3[:, :, 0], want3) and np.array_equal(g3[:, :, -1], want3)
 
 
 def test_argument_validation(cfm):

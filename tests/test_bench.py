@@ -36,5 +36,13 @@ def test_bench_json_line():
     assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
     assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3
     assert rf["traffic"] is None  # counted bytes exist only for the 512^3 x 1000-sweep workload
+    assert d["config"]["launch"] == "cooperative"  # a fallback launch is refused, not reported
     cb = d["cpu_baseline"]
     assert cb["kind"] == "port" and cb["value"] > 0 and cb["cores"] >= 1 and cb["sample"]
+    assert cb["host"]["nproc"] >= 1 and "model" in cb["host"]
+    c1 = cb["config1_2d_1024"]
+    assert set(c1) == {f"threads_{cb['cores']}", "threads_1"} or cb["cores"] == 1
+    for v in c1.values():
+        assert v["norms"] == 100 and v["glups"] > 0
+    cg = cb["cg_128_standard"]
+    assert cg["iterations"] > 100 and cg["rel_residual"] <= 1e-8

@@ -161,10 +161,10 @@ def test_cpu_configuration_1024_10000(gpu, orc, time_block):
     assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
 
 
-@pytest.mark.parametrize("dims,prec", [(3, "f64"), (2, "f64"), (3, "f32")])
+@pytest.mark.parametrize("dims,prec", [(3, "f64"), (2, "f64"), (3, "f32"), (2, "f32")])
 def test_benchmark_configuration_bitexact(gpu, orc, dims, prec):
     """The GPU benchmark configurations at full size against the oracle:
-    3D 512^3 and 2D 16384^2 FP64 and the 3D FP32 sweep (FP64 norms), sin x
+    3D 512^3 and 2D 16384^2, FP64 and the FP32 sweep (FP64 norms), sin x
     faces, Dirichlet, 1000 sweeps (two per HBM pass in 3D, four in 2D),
     update norm every 100 -- field bit for bit."""
     shape = (512, 512, 512) if dims == 3 else (16384, 16384)

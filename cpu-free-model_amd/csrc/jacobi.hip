@@ -1585,14 +1585,16 @@ static KernelFn pick_t4d_kernel(bool f64, bool aligned) {
                           : (KernelFn)jacobi2d_t4_kernel<float, false, 512>);
 }
 
+// (the FP64 1024-thread build for an unaligned row pitch needs more than the 128
+// VGPRs of four waves per SIMD and would spill to scratch: that shape runs 512)
 static KernelFn pick_t3d_kernel(bool f64, bool aligned, int block) {
     if (block == 512)
         return f64 ? (aligned ? (KernelFn)jacobi2d_t3_kernel<double, true, 512>
                               : (KernelFn)jacobi2d_t3_kernel<double, false, 512>)
                    : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 512>
                               : (KernelFn)jacobi2d_t3_kernel<float, false, 512>);
-    return f64 ? (aligned ? (KernelFn)jacobi2d_t3_kernel<double, true, 1024>
-                          : (KernelFn)jacobi2d_t3_kernel<double, false, 1024>)
+    if (f64 && !aligned) return nullptr;
+    return f64 ? (KernelFn)jacobi2d_t3_kernel<double, true, 1024>
                : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 1024>
                           : (KernelFn)jacobi2d_t3_kernel<float, false, 1024>);
 }
@@ -1611,9 +1613,11 @@ static KernelFn pick_t2d_kernel(bool f64, bool aligned, int block) {
 
 // Default workgroup size per tile shape: one workgroup fills a CU (fewer
 // workgroups make the device-wide barrier cheaper).  3D: WX = 4 wavefronts
-// across x and block/256 wavefronts down y share halos through LDS.
-static int default_block(bool is3d, int rows) {
-    if (is3d) return rows == 1 ? 256 : rows == 2 ? 1024 : rows == 4 ? 768 : 512;
+// across x and block/256 wavefronts down y share halos through LDS; 8-row
+// tiles run one tile per wavefront (256).  FP64 4-row tiles in 768 threads
+// would exceed the VGPRs of three waves per SIMD (scratch spills): 512.
+static int default_block(bool is3d, int rows, bool f64) {
+    if (is3d) return rows == 1 ? 256 : rows == 2 ? 1024 : rows == 4 ? (f64 ? 512 : 768) : 256;
     return rows == 4 ? 1024 : rows == 8 ? 768 : 512;
 }
 
@@ -1632,11 +1636,13 @@ static KernelFn pick_kernel_t(bool is3d, int rows, int block) {
                 default: return nullptr;
             }
         }
+        // shared-tile shapes that fit the register file without scratch spills: 2 rows in
+        // 1024 threads, 4 rows in 512, and (FP32 only) 4 rows in 768
         if (rows == 2 && block == 1024) return CFM_W3(2, 4);
-        if (rows == 4 && block == 1024) return CFM_W3(4, 4);
-        if (rows == 4 && block == 768) return CFM_W3(4, 3);
+        if constexpr (sizeof(T) == 4) {
+            if (rows == 4 && block == 768) return CFM_W3(4, 3);
+        }
         if (rows == 4 && block == 512) return CFM_W3(4, 2);
-        if (rows == 8 && block == 512) return CFM_W3(8, 2);
         return nullptr;
     }
     const bool wide = block != 256;
@@ -1658,9 +1664,11 @@ constexpr int kT2Width = 512;
 // threads) or, with 512 threads, 2 x 4 wavefronts of 4 columns per lane
 // (twice the registers per thread, half the per-wave overhead per point).
 // FP32: 2 x 8 wavefronts of 4 columns per lane.  All tiles are 512 wide.
-template <typename T, int WX, int WY, int VXT>
+template <typename T, int WX, int WY, int VXT, bool FULL_OK = true>
 static KernelFn pick_t2_shape(bool aligned, bool full) {
-    if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true>;
+    if constexpr (FULL_OK) {
+        if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true>;
+    }
     return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, false>
                    : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, false, false>;
 }
@@ -1668,7 +1676,9 @@ template <typename T>
 static KernelFn pick_t2_kernel(bool aligned, bool full, int block) {
     if constexpr (sizeof(T) == 8) {
         if (block == 512) return pick_t2_shape<T, 2, 4, 4>(aligned, full);
-        return pick_t2_shape<T, 4, 4, 2>(aligned, full);
+        // the full-width (nx = 512) build of this shape spills to scratch; the general
+        // aligned build handles nx = 512 as well
+        return pick_t2_shape<T, 4, 4, 2, false>(aligned, full);
     } else {
         if (block == 512) return pick_t2_shape<T, 1, 8, 8>(aligned, full);
         return pick_t2_shape<T, 2, 8, 4>(aligned, full);
@@ -1785,7 +1795,8 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
         pl->rows = 1;
         // 16384^2: FP64 617 GLUP/s at 1024 threads (2 columns per lane) vs 534 at 512;
         // FP32 1205 at 1024 threads vs 942 at 512
-        pl->block = p->block_threads > 0 ? p->block_threads : (pl->sweeps_per_pass == 4 ? 512 : 1024);
+        const bool wide_ok = !(pl->sweeps_per_pass == 3 && f64 && !aligned);  // (that build spills at 1024)
+        pl->block = p->block_threads > 0 ? p->block_threads : (pl->sweeps_per_pass == 4 || !wide_ok ? 512 : 1024);
         pl->kernel = pl->sweeps_per_pass == 4   ? pick_t4d_kernel(f64, aligned)
                      : pl->sweeps_per_pass == 3 ? pick_t3d_kernel(f64, aligned, pl->block)
                                                 : pick_t2d_kernel(f64, aligned, pl->block);
@@ -1800,8 +1811,8 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
                          : pick_t2_kernel<float>(aligned, full, pl->block);
     } else {
         pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
-        pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows);
-        if (!is3d && pl->block != 256 && pl->block != default_block(false, pl->rows)) {
+        pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows, f64);
+        if (!is3d && pl->block != 256 && pl->block != default_block(false, pl->rows, f64)) {
             delete pl;
             return CFM_EINVAL;
         }
@@ -1810,7 +1821,8 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     }
     if (!pl->kernel) {
         delete pl;
-        return CFM_EINVAL;
+        return CFM_FAIL(CFM_EINVAL, "jacobi: no kernel for %d-row tiles in %d-thread workgroups (%s, %s rows)",
+                        pl->rows, pl->block, f64 ? "FP64" : "FP32", aligned ? "16-byte aligned" : "unaligned");
     }
     // grid: all co-resident workgroups
     int per_cu = 0;

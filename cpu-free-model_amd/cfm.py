@@ -40,8 +40,10 @@ EXPORTS = (
     "cfm_unet_param_count", "cfm_unet_create", "cfm_unet_destroy", "cfm_unet_init_params", "cfm_unet_set_params",
     "cfm_unet_get_params", "cfm_unet_forward", "cfm_unet_gradient", "cfm_unet_train_step",
     "cfm_cg_batch_x0_rejected", "cfm_cg_batch_load_features", "cfm_cg_batch_warm_start", "cfm_gen_varcoef3d_coef",
-    "cfm_cg_batch_smooth_x0", "cfm_gen_tridiag", "cfm_mm_read", "cfm_host_free",
+    "cfm_cg_batch_smooth_x0", "cfm_gen_tridiag", "cfm_mm_read", "cfm_host_free", "cfm_jacobi_plan_phase_times",
+    "cfm_cg_plan_phase_times",
 )
+PHASES = ("spmv", "update", "reduce", "barrier", "other")
 
 
 class CfmError(RuntimeError):
@@ -54,7 +56,7 @@ class CfmError(RuntimeError):
 class JacobiParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "nx", "ny", "nz", "niter", "norm_every", "prec", "bc", "variant", "tile_rows", "grid_blocks",
-        "chunks", "spin_limit_us", "block_threads", "time_block")]
+        "chunks", "spin_limit_us", "block_threads", "time_block", "phase_timers")]
 
 
 class Timing(ctypes.Structure):
@@ -72,7 +74,16 @@ class CgParams(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("restart_quirk", ctypes.c_int), ("max_outer", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("grid_blocks", ctypes.c_int),
                 ("spin_limit_us", ctypes.c_int), ("block_threads", ctypes.c_int), ("x0_guard", ctypes.c_int),
-                ("matrix_format", ctypes.c_int)]
+                ("matrix_format", ctypes.c_int), ("phase_timers", ctypes.c_int)]
+
+
+class PhaseTimes(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 5), ("max_ms", ctypes.c_double * 5), ("total_ms", ctypes.c_double),
+                ("workgroups", ctypes.c_int), ("host_timed", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"ms": dict(zip(PHASES, self.ms)), "max_ms": dict(zip(PHASES, self.max_ms)),
+                "total_ms": self.total_ms, "workgroups": self.workgroups, "host_timed": bool(self.host_timed)}
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -163,6 +174,8 @@ def _load():
     L.cfm_cg_batch_warm_start.argtypes = [_vp, _vp, ctypes.c_double, ctypes.POINTER(Timing)]
     L.cfm_gen_varcoef3d_coef.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp]
     L.cfm_cg_batch_smooth_x0.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Timing)]
+    L.cfm_jacobi_plan_phase_times.argtypes = [_vp, ctypes.POINTER(PhaseTimes)]
+    L.cfm_cg_plan_phase_times.argtypes = [_vp, ctypes.POINTER(PhaseTimes)]
     L.cfm_gen_tridiag.argtypes = [ctypes.c_int, ctypes.c_uint, _vp, _vp, _vp]
     L.cfm_gen_tridiag.restype = ctypes.c_int64
     L.cfm_mm_read.argtypes = [ctypes.c_char_p, _ip, _ip, _ip, ctypes.POINTER(_ip), ctypes.POINTER(_ip),
@@ -304,9 +317,9 @@ def init_field(nx, ny, nz=1, kind=INIT_SIN, prec=F64):
 
 def jacobi_params(nx, ny, nz=1, niter=1000, norm_every=0, prec=F64, bc=BC_DIRICHLET,
                   variant=PERSISTENT, tile_rows=0, grid_blocks=0, chunks=0, spin_limit_us=0, block_threads=0,
-                  time_block=0):
+                  time_block=0, phase_timers=0):
     return JacobiParams(nx, ny, nz, niter, norm_every, prec, bc, variant, tile_rows, grid_blocks, chunks,
-                        spin_limit_us, block_threads, time_block)
+                        spin_limit_us, block_threads, time_block, phase_timers)
 
 
 class JacobiPlan:
@@ -340,6 +353,11 @@ class JacobiPlan:
         out = np.empty(self.shape, dtype=self.dtype)
         _check(lib.cfm_jacobi_plan_download(self._h, _ptr(out)), "cfm_jacobi_plan_download")
         return out
+
+    def phase_times(self):
+        pt = PhaseTimes()
+        _check(lib.cfm_jacobi_plan_phase_times(self._h, ctypes.byref(pt)), "cfm_jacobi_plan_phase_times")
+        return pt.as_dict()
 
     def close(self):
         if self._h:
@@ -432,9 +450,9 @@ def uniform(seed, stream_id, count):
 
 def cg_params(method=CG_STANDARD, mode=CG_PERSISTENT, tol=1e-8, max_iter=10000, restart_quirk=1,
               max_outer=0, inner_tol=0.0, grid_blocks=0, spin_limit_us=0, block_threads=0, x0_guard=0,
-              matrix_format=0):
+              matrix_format=0, phase_timers=0):
     return CgParams(method, mode, tol, max_iter, restart_quirk, max_outer, inner_tol, grid_blocks,
-                    spin_limit_us, block_threads, x0_guard, matrix_format)
+                    spin_limit_us, block_threads, x0_guard, matrix_format, phase_timers)
 
 
 class CgPlan:
@@ -495,6 +513,11 @@ class CgPlan:
         out = np.empty(self.n)
         _check(lib.cfm_cg_plan_get_x(self._h, out.ctypes.data_as(_dp)), "cfm_cg_plan_get_x")
         return out
+
+    def phase_times(self):
+        pt = PhaseTimes()
+        _check(lib.cfm_cg_plan_phase_times(self._h, ctypes.byref(pt)), "cfm_cg_plan_phase_times")
+        return pt.as_dict()
 
     def close(self):
         if self._h:

@@ -7,6 +7,8 @@
 // reference's rules (coordinate, real or integer, general / symmetric /
 // skew-symmetric; 0- or 1-based indices detected as CG/src/main.cu:180-186).
 // -json adds one JSON line per run (iterations, residual, times, launch kind).
+// -phase_timers prints each run's in-place time breakdown (spmv / update /
+// reduce / barrier / other); versions 6 and 7 always time their operations.
 // Right-hand side: -rhs uniform (SplitMix64 seed -seed, default) or -rhs ones
 // (the reference's b = 1).  -tol 0 runs exactly -niter iterations as the
 // reference does; otherwise the solve stops at ||r|| <= tol ||b||.
@@ -100,6 +102,7 @@ int main(int argc, char *argv[]) {
     const int grid = get_argval<int>(b, e, "-grid", 0);
     const int block = get_argval<int>(b, e, "-block", 0);
     const bool json = get_arg(b, e, "-json");
+    const bool phase_flag = get_arg(b, e, "-phase_timers");
 
     std::vector<int> versions;
     {
@@ -126,7 +129,12 @@ int main(int argc, char *argv[]) {
     std::vector<double> bvec(A.n, 1.0);
     if (rhs != "ones") cfm_gen_uniform(seed, 0, bvec.data(), A.n);
 
-    auto solve = [&](int version, std::vector<double> &x, cfm_timing &t, int &iters, std::vector<double> &hist) {
+    // versions 6 and 7 are the reference's "Profiling Discrete" ones: every operation is timed
+    // (stream synchronisation + a roctx range around it); -phase_timers asks any version for its
+    // in-place breakdown (the persistent kernels' own per-workgroup phase clocks)
+    auto timed = [&](int version) { return phase_flag || version == 6 || version == 7; };
+    auto solve = [&](int version, std::vector<double> &x, cfm_timing &t, int &iters, std::vector<double> &hist,
+                     cfm_phase_times *phases) {
         cfm_cg_params p{};
         p.method = methods[version];
         p.mode = modes[version];
@@ -135,10 +143,26 @@ int main(int argc, char *argv[]) {
         p.restart_quirk = quirk;
         p.grid_blocks = grid;
         p.block_threads = block;
+        p.phase_timers = timed(version) ? 1 : 0;
         x.assign(A.n, 0.0);
         hist.assign(iter_max + 2, 0.0);
-        return cfm_cg_solve(A.n, nnz, A.rowptr.data(), A.col.data(), A.val.data(), A.base, bvec.data(), x.data(),
-                            &p, hist.data(), (int)hist.size(), &iters, &t);
+        cfm_cg_plan *plan = nullptr;
+        int st = cfm_cg_plan_create(A.n, nnz, A.rowptr.data(), A.col.data(), A.val.data(), A.base, 0, &p, &plan);
+        if (st != CFM_OK) return st;
+        st = cfm_cg_plan_set_rhs(plan, bvec.data());
+        if (st == CFM_OK) st = cfm_cg_plan_set_x0(plan, nullptr);
+        int solve_st = CFM_OK;
+        if (st == CFM_OK) {
+            solve_st = cfm_cg_plan_solve(plan, hist.data(), (int)hist.size(), &iters, &t);
+            if (solve_st == CFM_OK || solve_st == CFM_ENOTCONV || solve_st == CFM_EBREAKDOWN) {
+                st = cfm_cg_plan_get_x(plan, x.data());
+                if (st == CFM_OK && phases && p.phase_timers) st = cfm_cg_plan_phase_times(plan, phases);
+            } else {
+                st = solve_st;
+            }
+        }
+        cfm_cg_plan_destroy(plan);
+        return st != CFM_OK ? st : solve_st;
     };
 
     std::vector<double> x_ref;
@@ -147,7 +171,7 @@ int main(int argc, char *argv[]) {
         cfm_timing t{};
         int it = 0;
         std::vector<double> h;
-        const int st = solve(8, x_ref, t, it, h);
+        const int st = solve(8, x_ref, t, it, h, nullptr);
         if (st != CFM_OK && st != CFM_ENOTCONV) return fail("single-GPU reference solve", st);
         single_runtime = t.wall_ms / 1e3;
     }
@@ -162,7 +186,8 @@ int main(int argc, char *argv[]) {
             std::vector<double> x, hist;
             cfm_timing t{};
             int iters = 0;
-            const int st = solve(v, x, t, iters, hist);
+            cfm_phase_times ph{};
+            const int st = solve(v, x, t, iters, hist, &ph);
             if (st != CFM_OK && st != CFM_ENOTCONV) return fail("CG solve", st);
             bool correct = true;
             if (cmp) {
@@ -187,12 +212,29 @@ int main(int argc, char *argv[]) {
                             "status: %s\n",
                             iters, t.outer_iterations, std::sqrt(hist[nh] / bb), t.kernel_ms, t.grid_blocks,
                             t.block_threads, cfm_status_str(st));
-                if (json)
+                static const char *phase_names[CFM_NPHASES] = {"spmv", "update", "reduce", "barrier", "other"};
+                if (timed(v)) {
+                    std::printf("phases (ms, %s):", ph.host_timed ? "host-timed operations"
+                                                                  : "mean over the kernel's workgroups");
+                    for (int i = 0; i < CFM_NPHASES; ++i) std::printf(" %s %.4f", phase_names[i], ph.ms[i]);
+                    std::printf(", total %.4f\n", ph.total_ms);
+                }
+                if (json) {
                     std::printf("{\"version\": \"%s\", \"iterations\": %d, \"outer_iterations\": %d, "
                                 "\"rel_residual\": %.6e, \"kernel_ms\": %.4f, \"wall_ms\": %.4f, \"grid\": %d, "
-                                "\"block\": %d, \"status\": \"%s\", \"cooperative\": %d, \"n\": %d, \"nnz\": %d}\n",
+                                "\"block\": %d, \"status\": \"%s\", \"cooperative\": %d, \"n\": %d, \"nnz\": %d",
                                 names[v], iters, t.outer_iterations, std::sqrt(hist[nh] / bb), t.kernel_ms,
                                 t.wall_ms, t.grid_blocks, t.block_threads, cfm_status_str(st), t.cooperative, A.n, nnz);
+                    if (timed(v)) {
+                        std::printf(", \"phases\": {\"host_timed\": %d, \"workgroups\": %d, \"total_ms\": %.4f",
+                                    ph.host_timed, ph.workgroups, ph.total_ms);
+                        for (int i = 0; i < CFM_NPHASES; ++i)
+                            std::printf(", \"%s_ms\": %.4f, \"%s_max_ms\": %.4f", phase_names[i], ph.ms[i],
+                                        phase_names[i], ph.max_ms[i]);
+                        std::printf("}");
+                    }
+                    std::printf("}\n");
+                }
             } else {
                 return 1;
             }

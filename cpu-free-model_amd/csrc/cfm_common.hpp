@@ -144,6 +144,67 @@ inline cfm_status barrier_timeout(const char *what, int cooperative) {
     return CFM_ETIMEOUT;
 }
 
+// -------------------------------------------------------- phase timers ----
+// Per-workgroup time per phase (CFM_PHASE_*), from the constant-rate
+// s_memrealtime clock read by thread 0 at every phase boundary: mark(p)
+// charges the time since the previous boundary to phase p.  The state lives
+// in the kernel's LDS (kClockWords words), so a timed build needs no extra
+// registers in its loops; with ON = false every member is empty, the LDS
+// array is unused and the timer code is compiled out.  flush() stores the
+// workgroup's CFM_NPHASES sums and its first-to-last span (ticks) at
+// out[blockIdx.x * kTimerSlots ...].
+constexpr int kTimerSlots = CFM_NPHASES + 1;
+constexpr int kClockWords = CFM_NPHASES + 2;
+
+template <bool ON>
+struct PhaseClock {
+    unsigned long long *s;  // LDS: [0] first reading, [1] last reading, [2 + phase] sums
+    __device__ __forceinline__ void start() {
+        if constexpr (ON) {
+            if (threadIdx.x == 0) {
+                s[0] = s[1] = __builtin_amdgcn_s_memrealtime();
+#pragma unroll
+                for (int i = 0; i < CFM_NPHASES; ++i) s[2 + i] = 0;
+            }
+        }
+    }
+    __device__ __forceinline__ void mark(int phase) {
+        if constexpr (ON) {
+            if (threadIdx.x == 0) {
+                const unsigned long long now = __builtin_amdgcn_s_memrealtime();
+                s[2 + phase] += now - s[1];
+                s[1] = now;
+            }
+        }
+    }
+    __device__ __forceinline__ void flush(unsigned long long *out) {
+        if constexpr (ON) {
+            if (out && threadIdx.x == 0) {
+                unsigned long long *o = out + (size_t)blockIdx.x * kTimerSlots;
+#pragma unroll
+                for (int i = 0; i < CFM_NPHASES; ++i) o[i] = s[2 + i];
+                o[CFM_NPHASES] = s[1] - s[0];
+            }
+        }
+    }
+};
+
+// Host: per-workgroup tick sums -> cfm_phase_times (ticks at rate_khz).
+inline void summarize_phases(const unsigned long long *ticks, int workgroups, double rate_khz, cfm_phase_times *out) {
+    *out = cfm_phase_times{};
+    if (workgroups < 1 || rate_khz <= 0.0) return;
+    for (int b = 0; b < workgroups; ++b) {
+        const unsigned long long *o = ticks + (size_t)b * kTimerSlots;
+        for (int i = 0; i < CFM_NPHASES; ++i) {
+            const double ms = (double)o[i] / rate_khz;
+            out->ms[i] += ms / workgroups;
+            if (ms > out->max_ms[i]) out->max_ms[i] = ms;
+        }
+        out->total_ms += (double)o[CFM_NPHASES] / rate_khz / workgroups;
+    }
+    out->workgroups = workgroups;
+}
+
 // --------------------------------------------------------- reductions ----
 // xor butterfly over the 64 lanes of a wavefront: offsets 32,16,8,4,2,1.
 // Every lane ends with the same value because a+b == b+a exactly.

@@ -34,6 +34,8 @@
 #include <cstring>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "cg_common.hpp"
 
 namespace cfm {
@@ -58,6 +60,7 @@ struct CgArgs {
     double inner_tol;
     int *info;           // [0] status, [1] iterations, [2] outer iterations
     GridSync *sync;
+    unsigned long long *timers;  // phase-timer build: [workgroup][kTimerSlots] ticks
 };
 
 
@@ -381,53 +384,77 @@ __device__ __forceinline__ void write_hist(const CgArgs &A, int k, double v) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && A.hist && k < A.hist_cap) A.hist[k] = v;
 }
 
-template <int SUB>
+// The persistent kernels.  TIMED builds a second copy that records every
+// workgroup's time per phase (PhaseClock); the default build has no timer code.
+template <int SUB, bool TIMED>
 __global__ void __launch_bounds__(256 * SUB) cg_standard_kernel(CgArgs A) {
     __shared__ double lds[4 * SUB];
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     unsigned int epoch = 0;
     StdPhases::init<SUB>(A, lds);
+    clk.mark(CFM_PHASE_OTHER);
     if (!grid_barrier(A.sync, epoch)) return;
+    clk.mark(CFM_PHASE_BARRIER);
     double gamma = reduce_partials<SUB>(A.part + 0 * A.nchunks, A.nchunks, lds);
     const double bb = reduce_partials<SUB>(A.part + 2 * A.nchunks, A.nchunks, lds);
     const double thr = A.tol > 0.0 ? A.tol * A.tol * bb : 0.0;
     write_hist(A, 0, gamma);
+    clk.mark(CFM_PHASE_REDUCE);
     int k = 0, status = CFM_OK;
     bool converged = gamma <= thr;
     while (!converged && k < A.max_iter) {
         StdPhases::spmv_dot<SUB>(A, lds);
+        clk.mark(CFM_PHASE_SPMV);
         if (!grid_barrier(A.sync, epoch)) return;
+        clk.mark(CFM_PHASE_BARRIER);
         const double delta = reduce_partials<SUB>(A.part + 1 * A.nchunks, A.nchunks, lds);
+        clk.mark(CFM_PHASE_REDUCE);
         if (!(delta > 0.0) || !is_finite(delta)) {
             status = CFM_EBREAKDOWN;
             break;
         }
         const double alpha = gamma / delta;
         StdPhases::update_xr<SUB>(A, alpha, lds);
+        clk.mark(CFM_PHASE_UPDATE);
         if (!grid_barrier(A.sync, epoch)) return;
+        clk.mark(CFM_PHASE_BARRIER);
         const double gamma_new = reduce_partials<SUB>(A.part + 0 * A.nchunks, A.nchunks, lds);
         ++k;
         write_hist(A, k, gamma_new);
+        clk.mark(CFM_PHASE_REDUCE);
         if (gamma_new <= thr) {
             converged = true;
             break;
         }
         const double beta = gamma_new / gamma;
         StdPhases::update_p<SUB>(A, beta);
+        clk.mark(CFM_PHASE_UPDATE);
         if (!grid_barrier(A.sync, epoch)) return;
+        clk.mark(CFM_PHASE_BARRIER);
         gamma = gamma_new;
     }
     if (status == CFM_OK && !converged && A.tol > 0.0) status = CFM_ENOTCONV;
     write_info(A, status, k, 0);
+    clk.flush(A.timers);
 }
 
-template <int SUB>
+template <int SUB, bool TIMED>
 __global__ void __launch_bounds__(256 * SUB) cg_pipelined_kernel(CgArgs A) {
     __shared__ double lds[4 * SUB];
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     unsigned int epoch = 0;
     PipePhases::init_r<SUB>(A, lds);
+    clk.mark(CFM_PHASE_OTHER);
     if (!grid_barrier(A.sync, epoch)) return;
+    clk.mark(CFM_PHASE_BARRIER);
     PipePhases::init_w<SUB>(A, lds);
+    clk.mark(CFM_PHASE_OTHER);
     if (!grid_barrier(A.sync, epoch)) return;
+    clk.mark(CFM_PHASE_BARRIER);
     const double bb = reduce_partials<SUB>(A.part + 4 * A.nchunks, A.nchunks, lds);
     const double thr = A.tol > 0.0 ? A.tol * A.tol * bb : 0.0;
     double alpha = 0.0, beta = 0.0, delta_prev = 0.0;
@@ -438,6 +465,7 @@ __global__ void __launch_bounds__(256 * SUB) cg_pipelined_kernel(CgArgs A) {
         const double delta = reduce_partials<SUB>(A.part + (0 + buf) * A.nchunks, A.nchunks, lds);
         const double gam = reduce_partials<SUB>(A.part + (2 + buf) * A.nchunks, A.nchunks, lds);
         write_hist(A, k, delta);
+        clk.mark(CFM_PHASE_REDUCE);
         if (delta <= thr) {
             converged = true;
             break;
@@ -457,30 +485,39 @@ __global__ void __launch_bounds__(256 * SUB) cg_pipelined_kernel(CgArgs A) {
         }
         alpha = delta / denom;
         PipePhases::iterate<SUB>(A, buf, alpha, beta, lds);
+        clk.mark(CFM_PHASE_SPMV);
         if (!grid_barrier(A.sync, epoch)) return;
+        clk.mark(CFM_PHASE_BARRIER);
         delta_prev = delta;
         ++k;
     }
     if (status == CFM_OK && !converged && A.tol > 0.0) status = CFM_ENOTCONV;
     write_info(A, status, k, 0);
+    clk.flush(A.timers);
 }
 
-template <int SUB>
+template <int SUB, bool TIMED>
 __global__ void __launch_bounds__(256 * SUB) cg_ir_kernel(CgArgs A) {
     __shared__ double lds[4 * SUB];
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     unsigned int epoch = 0;
     double thr = 0.0;
     int outer = 0, inner_total = 0, status = CFM_ENOTCONV;
     for (;;) {
         const bool first = outer == 0;
         IrPhases::outer<SUB>(A, first, lds);
+        clk.mark(CFM_PHASE_OTHER);
         if (!grid_barrier(A.sync, epoch)) return;
+        clk.mark(CFM_PHASE_BARRIER);
         const double rr = reduce_partials<SUB>(A.part + 0 * A.nchunks, A.nchunks, lds);
         if (first) {
             const double bb = reduce_partials<SUB>(A.part + 3 * A.nchunks, A.nchunks, lds);
             thr = A.tol * A.tol * bb;
         }
         write_hist(A, outer, rr);
+        clk.mark(CFM_PHASE_REDUCE);
         if (rr <= thr) {
             status = CFM_OK;
             break;
@@ -489,30 +526,41 @@ __global__ void __launch_bounds__(256 * SUB) cg_ir_kernel(CgArgs A) {
         double g = reduce_partials<SUB>(A.part + 1 * A.nchunks, A.nchunks, lds);
         // no refinement is solved beyond half the outer target
         const double thr_in = fmax(A.inner_tol * A.inner_tol * g, 0.25 * thr);
+        clk.mark(CFM_PHASE_REDUCE);
         int j = 0;
         bool brk = false;
         while (j < A.max_iter) {
             IrPhases::spmv_dot<SUB>(A, lds);
+            clk.mark(CFM_PHASE_SPMV);
             if (!grid_barrier(A.sync, epoch)) return;
+            clk.mark(CFM_PHASE_BARRIER);
             const double delta = reduce_partials<SUB>(A.part + 2 * A.nchunks, A.nchunks, lds);
+            clk.mark(CFM_PHASE_REDUCE);
             if (!(delta > 0.0) || !is_finite(delta)) {
                 brk = true;
                 break;
             }
             const float af = (float)(g / delta);
             IrPhases::update_dr<SUB>(A, af, lds);
+            clk.mark(CFM_PHASE_UPDATE);
             if (!grid_barrier(A.sync, epoch)) return;
+            clk.mark(CFM_PHASE_BARRIER);
             const double gn = reduce_partials<SUB>(A.part + 1 * A.nchunks, A.nchunks, lds);
+            clk.mark(CFM_PHASE_REDUCE);
             ++j;
             if (gn <= thr_in) break;
             const float bf = (float)(gn / g);
             IrPhases::update_p<SUB>(A, bf);
+            clk.mark(CFM_PHASE_UPDATE);
             if (!grid_barrier(A.sync, epoch)) return;
+            clk.mark(CFM_PHASE_BARRIER);
             g = gn;
         }
         inner_total += j;
         IrPhases::update_x<SUB>(A);
+        clk.mark(CFM_PHASE_UPDATE);
         if (!grid_barrier(A.sync, epoch)) return;
+        clk.mark(CFM_PHASE_BARRIER);
         ++outer;
         if (brk && j == 0) {
             status = CFM_EBREAKDOWN;
@@ -520,6 +568,7 @@ __global__ void __launch_bounds__(256 * SUB) cg_ir_kernel(CgArgs A) {
         }
     }
     write_info(A, status, inner_total, outer);
+    clk.flush(A.timers);
 }
 
 // ===================================================== discrete kernels ===
@@ -581,6 +630,7 @@ using CgKernelFn = void (*)(CgArgs);
 // The kernels of one workgroup shape (SUB sub-blocks of 256 threads).
 struct CgKernels {
     CgKernelFn standard, pipelined, ir;
+    CgKernelFn standard_timed, pipelined_timed, ir_timed;
     void (*std_init)(CgArgs);
     void (*std_spmv_dot)(CgArgs);
     void (*std_update_xr)(CgArgs, double);
@@ -589,17 +639,20 @@ struct CgKernels {
     void (*pipe_init_w)(CgArgs);
     void (*pipe_iterate)(CgArgs, int, double, double);
     void (*reduce)(CgArgs, int, int, double *);
-    CgKernelFn persistent(int method) const {
+    CgKernelFn persistent(int method, bool timed) const {
+        if (timed)
+            return method == CFM_CG_STANDARD ? standard_timed : method == CFM_CG_PIPELINED ? pipelined_timed : ir_timed;
         return method == CFM_CG_STANDARD ? standard : method == CFM_CG_PIPELINED ? pipelined : ir;
     }
 };
 
 template <int SUB>
 static CgKernels kernels_for() {
-    return CgKernels{cg_standard_kernel<SUB>, cg_pipelined_kernel<SUB>, cg_ir_kernel<SUB>,
-                     k_std_init<SUB>,         k_std_spmv_dot<SUB>,      k_std_update_xr<SUB>,
-                     k_std_update_p<SUB>,     k_pipe_init_r<SUB>,       k_pipe_init_w<SUB>,
-                     k_pipe_iterate<SUB>,     k_reduce<SUB>};
+    return CgKernels{cg_standard_kernel<SUB, false>, cg_pipelined_kernel<SUB, false>, cg_ir_kernel<SUB, false>,
+                     cg_standard_kernel<SUB, true>,  cg_pipelined_kernel<SUB, true>,  cg_ir_kernel<SUB, true>,
+                     k_std_init<SUB>,                k_std_spmv_dot<SUB>,             k_std_update_xr<SUB>,
+                     k_std_update_p<SUB>,            k_pipe_init_r<SUB>,              k_pipe_init_w<SUB>,
+                     k_pipe_iterate<SUB>,            k_reduce<SUB>};
 }
 
 static bool kernel_table(int sub, CgKernels *out) {
@@ -641,6 +694,9 @@ struct cfm_cg_plan {
     int sub = 4;  // 256-thread sub-blocks per workgroup
     CgKernels K{};
     unsigned long long spin_ticks = 0;
+    double clock_khz = 100000.0;           // s_memrealtime rate
+    unsigned long long *timers = nullptr;  // phase_timers: [grid][kTimerSlots] ticks
+    cfm_phase_times phases{};              // breakdown of the last solve
 };
 
 // Build SELL-64 on the host from CSR (rows padded to the slice width, col -1).
@@ -715,7 +771,7 @@ extern "C" void cfm_cg_plan_destroy(cfm_cg_plan *pl) {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     void *ptrs[] = {pl->slice_ptr, pl->scol, pl->sval, pl->sval32, pl->doff, pl->b, pl->x0, pl->x,
-                    pl->part, pl->hist, pl->scal, pl->info, pl->sync};
+                    pl->part, pl->hist, pl->scal, pl->info, pl->sync, pl->timers};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     for (double *q : pl->v)
@@ -858,7 +914,8 @@ extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, cons
     if (bt % kThreads != 0 || !kernel_table(bt / kThreads, &pl->K)) return fail(CFM_EINVAL);
     pl->sub = bt / kThreads;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pl->K.persistent(p->method), bt, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pl->K.persistent(p->method, p->phase_timers),
+                                                     bt, 0) !=
             hipSuccess ||
         per_cu < 1)
         return fail(CFM_EDEVICE);
@@ -871,6 +928,10 @@ extern "C" cfm_status cfm_cg_plan_create(int n, int nnz, const int *rowptr, cons
         rate_khz <= 0)
         rate_khz = 100000;
     pl->spin_ticks = us * (unsigned long long)rate_khz / 1000ull;
+    pl->clock_khz = rate_khz;
+    if (p->phase_timers && p->mode == CFM_CG_PERSISTENT &&
+        !dalloc((void **)&pl->timers, sizeof(unsigned long long) * kTimerSlots * pl->grid))
+        return fail(CFM_ENOMEM);
     *out = pl;
     return CFM_OK;
 }
@@ -922,18 +983,50 @@ static CgArgs make_args(cfm_cg_plan *pl) {
 }
 
 // Host-driven schedule of the same phases (reference "Baseline Discrete").
+// With phase_timers, every operation is bracketed by a roctx range and a
+// stream synchronisation and its host time is charged to its phase, as the
+// reference's profiling versions bracket theirs with NVTX ranges
+// (CG/src/profiling/discrete-standard.cu:222-264).
+struct OpTimer {
+    bool on;
+    hipStream_t s;
+    double ms[CFM_NPHASES] = {};
+    std::chrono::steady_clock::time_point t0;
+    void begin(const char *name) {
+        if (!on) return;
+        roctxRangePush(name);
+        t0 = std::chrono::steady_clock::now();
+    }
+    cfm_status end(int phase) {
+        if (!on) return CFM_OK;
+        CFM_HIP(hipStreamSynchronize(s));
+        ms[phase] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        roctxRangePop();
+        return CFM_OK;
+    }
+};
+
 static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &hist, int *iters_out,
-                               int *status_out) {
+                               int *status_out, OpTimer &ot) {
     const dim3 grid(pl->grid), block(kThreads * pl->sub);
     const CgKernels &K = pl->K;
     hipStream_t s = pl->stream;
     auto fetch = [&](int i0, int i1, double *a, double *b) -> cfm_status {
+        ot.begin("Dot (reduce + copy to host)");
         CFM_HIP(launch(K.reduce, dim3(1), block, s, A, i0, i1, pl->scal));
         CFM_HIP(hipMemcpyAsync(pl->scal_h, pl->scal, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
         CFM_HIP(hipStreamSynchronize(s));
+        cfm_status st_ = ot.end(CFM_PHASE_REDUCE);
+        if (st_ != CFM_OK) return st_;
         *a = pl->scal_h[0];
         if (b) *b = pl->scal_h[1];
         return CFM_OK;
+    };
+    // launch one operation of the schedule, timed when asked
+    auto op = [&](const char *name, int phase, auto launcher) -> cfm_status {
+        ot.begin(name);
+        CFM_HIP(launcher());
+        return ot.end(phase);
     };
     const int max_iter = pl->p.max_iter;
     const double tol = pl->p.tol;
@@ -941,14 +1034,18 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
     bool converged = false;
     cfm_status st;
     if (pl->p.method == CFM_CG_STANDARD) {
-        CFM_HIP(launch(K.std_init, grid, block, s, A));
+        if ((st = op("Init (r = b - A x0, p = r)", CFM_PHASE_OTHER,
+                     [&] { return launch(K.std_init, grid, block, s, A); })) != CFM_OK)
+            return st;
         double gamma, bb;
         if ((st = fetch(0, 2, &gamma, &bb)) != CFM_OK) return st;
         const double thr = tol > 0.0 ? tol * tol * bb : 0.0;
         hist.push_back(gamma);
         converged = gamma <= thr;
         while (!converged && k < max_iter) {
-            CFM_HIP(launch(K.std_spmv_dot, grid, block, s, A));
+            if ((st = op("SpMV (s = A p) + dot <p,s>", CFM_PHASE_SPMV,
+                         [&] { return launch(K.std_spmv_dot, grid, block, s, A); })) != CFM_OK)
+                return st;
             double delta;
             if ((st = fetch(1, -1, &delta, nullptr)) != CFM_OK) return st;
             if (!(delta > 0.0) || !std::isfinite(delta)) {
@@ -956,7 +1053,9 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
                 break;
             }
             const double alpha = gamma / delta;
-            CFM_HIP(launch(K.std_update_xr, grid, block, s, A, alpha));
+            if ((st = op("Saxpy (x += alpha p, r -= alpha s) + dot <r,r>", CFM_PHASE_UPDATE,
+                         [&] { return launch(K.std_update_xr, grid, block, s, A, alpha); })) != CFM_OK)
+                return st;
             double gamma_new;
             if ((st = fetch(0, -1, &gamma_new, nullptr)) != CFM_OK) return st;
             ++k;
@@ -966,12 +1065,18 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
                 break;
             }
             const double beta = gamma_new / gamma;
-            CFM_HIP(launch(K.std_update_p, grid, block, s, A, beta));
+            if ((st = op("Scale and saxpy (p = r + beta p)", CFM_PHASE_UPDATE,
+                         [&] { return launch(K.std_update_p, grid, block, s, A, beta); })) != CFM_OK)
+                return st;
             gamma = gamma_new;
         }
     } else {
-        CFM_HIP(launch(K.pipe_init_r, grid, block, s, A));
-        CFM_HIP(launch(K.pipe_init_w, grid, block, s, A));
+        if ((st = op("Init (r = b - A x0)", CFM_PHASE_OTHER,
+                     [&] { return launch(K.pipe_init_r, grid, block, s, A); })) != CFM_OK)
+            return st;
+        if ((st = op("Init (w = A r)", CFM_PHASE_OTHER,
+                     [&] { return launch(K.pipe_init_w, grid, block, s, A); })) != CFM_OK)
+            return st;
         double bb, dummy;
         if ((st = fetch(4, -1, &bb, nullptr)) != CFM_OK) return st;
         (void)dummy;
@@ -1000,7 +1105,9 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
                 break;
             }
             alpha = delta / denom;
-            CFM_HIP(launch(K.pipe_iterate, grid, block, s, A, buf, alpha, beta));
+            if ((st = op("SpMV (q = A w) + dots + vector updates", CFM_PHASE_SPMV,
+                         [&] { return launch(K.pipe_iterate, grid, block, s, A, buf, alpha, beta); })) != CFM_OK)
+                return st;
             delta_prev = delta;
             ++k;
         }
@@ -1012,9 +1119,30 @@ static cfm_status run_discrete(cfm_cg_plan *pl, CgArgs &A, std::vector<double> &
     return CFM_OK;
 }
 
+// A roctx range around every plan call (a no-op unless a profiler listens).
+struct RoctxRange {
+    explicit RoctxRange(const char *name) { roctxRangePush(name); }
+    ~RoctxRange() { roctxRangePop(); }
+};
+
+static const char *solve_range_name(const cfm_cg_params &p) {
+    static const char *names[3][2] = {{"cfm_cg_plan_solve standard persistent", "cfm_cg_plan_solve standard discrete"},
+                                      {"cfm_cg_plan_solve pipelined persistent", "cfm_cg_plan_solve pipelined discrete"},
+                                      {"cfm_cg_plan_solve ir_f32 persistent", "cfm_cg_plan_solve ir_f32 persistent"}};
+    return names[p.method][p.mode == CFM_CG_DISCRETE ? 1 : 0];
+}
+
+extern "C" cfm_status cfm_cg_plan_phase_times(cfm_cg_plan *pl, cfm_phase_times *out) {
+    if (!pl || !out) return CFM_EINVAL;
+    if (!pl->p.phase_timers) return CFM_FAIL(CFM_EINVAL, "cg: the plan was created without phase_timers");
+    *out = pl->phases;
+    return CFM_OK;
+}
+
 extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_cap, int *iters,
                                         cfm_timing *t) {
     if (!pl || hist_cap < 0) return CFM_EINVAL;
+    RoctxRange range(solve_range_name(pl->p));
     const auto w0 = std::chrono::steady_clock::now();
     // history buffer sized to the request
     const int cap = hist ? hist_cap : 0;
@@ -1028,21 +1156,30 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
     CgArgs A = make_args(pl);
     A.hist_cap = cap;
     if (!hist) A.hist = nullptr;
+    A.timers = pl->timers;
     int status = CFM_OK, k = 0, outer = 0, coop = -1;
     float ms = 0.f;
+    const bool timed = pl->p.phase_timers != 0;
+    pl->phases = cfm_phase_times{};
     if (pl->p.mode == CFM_CG_PERSISTENT) {
         hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
         CFM_HIP(hipGetLastError());
         CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
         void *args[] = {&A};
-        CFM_HIP(launch_persistent((const void *)pl->K.persistent(pl->p.method), pl->grid, kThreads * pl->sub, args,
-                                  pl->stream, &coop));
+        CFM_HIP(launch_persistent((const void *)pl->K.persistent(pl->p.method, timed), pl->grid, kThreads * pl->sub,
+                                  args, pl->stream, &coop));
         CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
         CFM_HIP(hipEventSynchronize(pl->ev1));
         CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));
         GridSync hs{};
         CFM_HIP(hipMemcpy(&hs, pl->sync, sizeof(hs), hipMemcpyDeviceToHost));
         if (hs.abort) return barrier_timeout("cg", coop);
+        if (timed) {
+            std::vector<unsigned long long> ticks((size_t)kTimerSlots * pl->grid);
+            CFM_HIP(hipMemcpy(ticks.data(), pl->timers, sizeof(unsigned long long) * ticks.size(),
+                              hipMemcpyDeviceToHost));
+            summarize_phases(ticks.data(), pl->grid, pl->clock_khz, &pl->phases);
+        }
         int info[4] = {0, 0, 0, 0};
         CFM_HIP(hipMemcpy(info, pl->info, sizeof(int) * 4, hipMemcpyDeviceToHost));
         status = info[0];
@@ -1054,9 +1191,18 @@ extern "C" cfm_status cfm_cg_plan_solve(cfm_cg_plan *pl, double *hist, int hist_
         }
     } else {
         std::vector<double> h;
+        OpTimer ot{timed, pl->stream};
+        const auto h0 = std::chrono::steady_clock::now();
         CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
-        cfm_status st = run_discrete(pl, A, h, &k, &status);
+        cfm_status st = run_discrete(pl, A, h, &k, &status, ot);
         if (st != CFM_OK) return st;
+        if (timed) {
+            for (int i = 0; i < CFM_NPHASES; ++i) pl->phases.ms[i] = pl->phases.max_ms[i] = ot.ms[i];
+            pl->phases.total_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+            pl->phases.workgroups = 1;
+            pl->phases.host_timed = 1;
+        }
         CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
         CFM_HIP(hipEventSynchronize(pl->ev1));
         CFM_HIP(hipEventElapsedTime(&ms, pl->ev0, pl->ev1));

@@ -31,7 +31,9 @@
 #include <cstring>
 #include <vector>
 
-#include "cfm_common.hpp"
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include "jacobi_host.hpp"
 
 namespace cfm {
 
@@ -45,21 +47,17 @@ __global__ void reset_sync_kernel(GridSync *s, unsigned long long limit_ticks, i
     for (int b = t; b < grid; b += gridDim.x * blockDim.x) s->arrive[b] = 0;
 }
 
-struct JacobiArgs {
-    void *buf0;
-    void *buf1;
-    int nx, ny, nz;
-    int niter;        // sweeps in this launch
-    int it0;          // index of the first sweep (parity and norm slots)
-    int bc;
-    int norm_every;
-    int chunks;       // chunks along the streamed axis
-    int chunk_len;
-    int ntx, nty;     // tiles along x and y (nty = 1 in 2D)
-    int no_compute;
-    double *norm_part;  // [slot][gridDim.x]
-    GridSync *sync;
-};
+// ------------------------------------------------------------ pass clock --
+// Phase timers of the Jacobi kernels (TIMED builds, A.timers set): the time
+// between barriers goes to the sweeps, the wait at the device-wide barrier to
+// the barrier.
+template <bool TIMED>
+__device__ __forceinline__ bool pass_barrier(const JacobiArgs &A, unsigned int &epoch, PhaseClock<TIMED> &clk) {
+    clk.mark(CFM_PHASE_SPMV);
+    const bool ok = grid_barrier(A.sync, epoch);
+    clk.mark(CFM_PHASE_BARRIER);
+    return ok;
+}
 
 typedef double v2f64 __attribute__((ext_vector_type(2)));
 typedef float v4f32 __attribute__((ext_vector_type(4)));
@@ -312,7 +310,7 @@ __device__ __forceinline__ void sweep3d_tile(const T *__restrict__ src, T *__res
     }
 }
 
-template <typename T, int RY, bool ALIGNED, int BLOCK>
+template <typename T, int RY, bool ALIGNED, int BLOCK, bool TIMED>
 __global__ void __launch_bounds__(BLOCK) jacobi3d_kernel(JacobiArgs A) {
     __shared__ double lds[BLOCK / 64];
     const int wave_id = xcd_wave_id();
@@ -320,6 +318,9 @@ __global__ void __launch_bounds__(BLOCK) jacobi3d_kernel(JacobiArgs A) {
     const int ntiles = A.ntx * A.nty;
     const int nunits = ntiles * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     for (int k = 0; k < A.niter; ++k) {
         const int it = A.it0 + k;
         const bool want = A.norm_every > 0 && (it % A.norm_every) == 0;
@@ -340,9 +341,11 @@ __global__ void __launch_bounds__(BLOCK) jacobi3d_kernel(JacobiArgs A) {
             }
         }
         if (k + 1 < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // ------------------------------------------------------- 3D, workgroup tile --
@@ -550,7 +553,7 @@ __device__ __forceinline__ void sweep3d_wgtile(const T *__restrict__ src, T *__r
     __syncthreads();  // the LDS planes are reused by the next unit
 }
 
-template <typename T, int RY, int WX, int WY, bool ALIGNED>
+template <typename T, int RY, int WX, int WY, bool ALIGNED, bool TIMED>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A) {
     using Tl = WgTile<T, RY, WX, WY>;
     __shared__ typename Tl::Plane planes[2];
@@ -560,6 +563,9 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A)
     const int ntiles = A.ntx * A.nty;
     const int nunits = ntiles * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     for (int k = 0; k < A.niter; ++k) {
         const int it = A.it0 + k;
         const bool want = A.norm_every > 0 && (it % A.norm_every) == 0;
@@ -580,9 +586,11 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A)
             }
         }
         if (k + 1 < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // ------------------------------------------------- 3D, two sweeps per pass --
@@ -826,7 +834,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     __syncthreads();  // the LDS buffers are reused by the next unit
 }
 
-template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL>
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool TIMED>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
     using Tl = T2Tile<T, WX, WY, VXT>;
     __shared__ typename Tl::Buf bufs[3];
@@ -835,6 +843,9 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
     const int lb = (g & 7) == 0 ? (b & 7) * (g >> 3) + (b >> 3) : b;  // XCD-contiguous order
     const int nunits = A.nty * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     int pass = 0;
     for (int k = 0; k < A.niter; ++pass) {
         const int steps = min(2, A.niter - k);
@@ -859,9 +870,11 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
         }
         k += steps;
         if (k < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // ------------------------------------------------------------------ 2D ----
@@ -966,13 +979,16 @@ __device__ __forceinline__ void sweep2d_tile(const T *__restrict__ src, T *__res
     }
 }
 
-template <typename T, int RB, bool ALIGNED, int BLOCK>
+template <typename T, int RB, bool ALIGNED, int BLOCK, bool TIMED>
 __global__ void __launch_bounds__(BLOCK) jacobi2d_kernel(JacobiArgs A) {
     __shared__ double lds[BLOCK / 64];
     const int wave_id = xcd_wave_id();
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     const int nunits = A.ntx * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     for (int k = 0; k < A.niter; ++k) {
         const int it = A.it0 + k;
         const bool want = A.norm_every > 0 && (it % A.norm_every) == 0;
@@ -991,9 +1007,11 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_kernel(JacobiArgs A) {
             }
         }
         if (k + 1 < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // ------------------------------------------------- 2D, two sweeps per pass --
@@ -1157,7 +1175,7 @@ __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restr
     }
 }
 
-template <typename T, bool ALIGNED, int BLOCK>
+template <typename T, bool ALIGNED, int BLOCK, bool TIMED>
 __global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
     constexpr int VX = t2d_cols<T>(BLOCK);
     __shared__ double lds[BLOCK / 64];
@@ -1165,6 +1183,9 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     const int nunits = A.ntx * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     int pass = 0;
     for (int k = 0; k < A.niter; ++pass) {
         const int steps = min(2, A.niter - k);
@@ -1189,9 +1210,11 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
         }
         k += steps;
         if (k < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // Three sweeps per pass in 2D: as sweep2d_t2 with one more level.  At step r
@@ -1342,7 +1365,7 @@ __device__ __forceinline__ void sweep2d_t3(const T *__restrict__ src, T *__restr
     }
 }
 
-template <typename T, bool ALIGNED, int BLOCK>
+template <typename T, bool ALIGNED, int BLOCK, bool TIMED>
 __global__ void __launch_bounds__(BLOCK) jacobi2d_t3_kernel(JacobiArgs A) {
     constexpr int VX = t2d_cols<T>(BLOCK);
     __shared__ double lds[BLOCK / 64];
@@ -1350,6 +1373,9 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t3_kernel(JacobiArgs A) {
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     const int nunits = A.ntx * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     int pass = 0;
     for (int k = 0; k < A.niter; ++pass) {
         const int steps = min(3, A.niter - k);
@@ -1377,9 +1403,11 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t3_kernel(JacobiArgs A) {
         }
         k += steps;
         if (k < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // L sweeps per pass in 2D, the general form of sweep2d_t3: at step r level j
@@ -1526,7 +1554,7 @@ struct Sweep2dN {
     }
 };
 
-template <typename T, bool ALIGNED, int BLOCK>
+template <typename T, bool ALIGNED, int BLOCK, bool TIMED>
 __global__ void __launch_bounds__(BLOCK) jacobi2d_t4_kernel(JacobiArgs A) {
     constexpr int VX = t2d_cols<T>(BLOCK);
     __shared__ double lds[BLOCK / 64];
@@ -1534,6 +1562,9 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t4_kernel(JacobiArgs A) {
     const int nwaves = gridDim.x * (blockDim.x >> 6);
     const int nunits = A.ntx * A.chunks;
     unsigned int epoch = 0;
+    __shared__ unsigned long long clk_lds[kClockWords];
+    PhaseClock<TIMED> clk{clk_lds};
+    clk.start();
     int pass = 0;
     for (int k = 0; k < A.niter; ++pass) {
         const int steps = min(4, A.niter - k);
@@ -1569,46 +1600,50 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t4_kernel(JacobiArgs A) {
         }
         k += steps;
         if (k < A.niter) {
-            if (!grid_barrier(A.sync, epoch)) return;
+            if (!pass_barrier(A, epoch, clk)) return;
         }
     }
+    clk.mark(CFM_PHASE_SPMV);
+    clk.flush(A.timers);
 }
 
 // ------------------------------------------------------------- dispatch ---
-using KernelFn = void (*)(JacobiArgs);
 
 // four sweeps per pass: 512-thread workgroups only (the 1024-thread build spills)
+template <bool TM>
 static KernelFn pick_t4d_kernel(bool f64, bool aligned) {
-    return f64 ? (aligned ? (KernelFn)jacobi2d_t4_kernel<double, true, 512>
-                          : (KernelFn)jacobi2d_t4_kernel<double, false, 512>)
-               : (aligned ? (KernelFn)jacobi2d_t4_kernel<float, true, 512>
-                          : (KernelFn)jacobi2d_t4_kernel<float, false, 512>);
+    return f64 ? (aligned ? (KernelFn)jacobi2d_t4_kernel<double, true, 512, TM>
+                          : (KernelFn)jacobi2d_t4_kernel<double, false, 512, TM>)
+               : (aligned ? (KernelFn)jacobi2d_t4_kernel<float, true, 512, TM>
+                          : (KernelFn)jacobi2d_t4_kernel<float, false, 512, TM>);
 }
 
 // (the FP64 1024-thread build for an unaligned row pitch needs more than the 128
 // VGPRs of four waves per SIMD and would spill to scratch: that shape runs 512)
+template <bool TM>
 static KernelFn pick_t3d_kernel(bool f64, bool aligned, int block) {
     if (block == 512)
-        return f64 ? (aligned ? (KernelFn)jacobi2d_t3_kernel<double, true, 512>
-                              : (KernelFn)jacobi2d_t3_kernel<double, false, 512>)
-                   : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 512>
-                              : (KernelFn)jacobi2d_t3_kernel<float, false, 512>);
+        return f64 ? (aligned ? (KernelFn)jacobi2d_t3_kernel<double, true, 512, TM>
+                              : (KernelFn)jacobi2d_t3_kernel<double, false, 512, TM>)
+                   : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 512, TM>
+                              : (KernelFn)jacobi2d_t3_kernel<float, false, 512, TM>);
     if (f64 && !aligned) return nullptr;
-    return f64 ? (KernelFn)jacobi2d_t3_kernel<double, true, 1024>
-               : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 1024>
-                          : (KernelFn)jacobi2d_t3_kernel<float, false, 1024>);
+    return f64 ? (KernelFn)jacobi2d_t3_kernel<double, true, 1024, TM>
+               : (aligned ? (KernelFn)jacobi2d_t3_kernel<float, true, 1024, TM>
+                          : (KernelFn)jacobi2d_t3_kernel<float, false, 1024, TM>);
 }
 
+template <bool TM>
 static KernelFn pick_t2d_kernel(bool f64, bool aligned, int block) {
     if (block == 512)
-        return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 512>
-                              : (KernelFn)jacobi2d_t2_kernel<double, false, 512>)
-                   : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 512>
-                              : (KernelFn)jacobi2d_t2_kernel<float, false, 512>);
-    return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 1024>
-                          : (KernelFn)jacobi2d_t2_kernel<double, false, 1024>)
-               : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 1024>
-                          : (KernelFn)jacobi2d_t2_kernel<float, false, 1024>);
+        return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 512, TM>
+                              : (KernelFn)jacobi2d_t2_kernel<double, false, 512, TM>)
+                   : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 512, TM>
+                              : (KernelFn)jacobi2d_t2_kernel<float, false, 512, TM>);
+    return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 1024, TM>
+                          : (KernelFn)jacobi2d_t2_kernel<double, false, 1024, TM>)
+               : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 1024, TM>
+                          : (KernelFn)jacobi2d_t2_kernel<float, false, 1024, TM>);
 }
 
 // Default workgroup size per tile shape: one workgroup fills a CU (fewer
@@ -1621,11 +1656,11 @@ static int default_block(bool is3d, int rows, bool f64) {
     return rows == 4 ? 1024 : rows == 8 ? 768 : 512;
 }
 
-template <typename T, bool AL>
+template <typename T, bool AL, bool TM>
 static KernelFn pick_kernel_t(bool is3d, int rows, int block) {
-#define CFM_J3(R, B) (KernelFn)jacobi3d_kernel<T, R, AL, B>
-#define CFM_W3(R, WY) (KernelFn)jacobi3d_wg_kernel<T, R, 4, WY, AL>
-#define CFM_J2(R, B) (KernelFn)jacobi2d_kernel<T, R, AL, B>
+#define CFM_J3(R, B) (KernelFn)jacobi3d_kernel<T, R, AL, B, TM>
+#define CFM_W3(R, WY) (KernelFn)jacobi3d_wg_kernel<T, R, 4, WY, AL, TM>
+#define CFM_J2(R, B) (KernelFn)jacobi2d_kernel<T, R, AL, B, TM>
     if (is3d) {
         if (block == 256) {  // one independent tile per wavefront
             switch (rows) {
@@ -1664,30 +1699,31 @@ constexpr int kT2Width = 512;
 // threads) or, with 512 threads, 2 x 4 wavefronts of 4 columns per lane
 // (twice the registers per thread, half the per-wave overhead per point).
 // FP32: 2 x 8 wavefronts of 4 columns per lane.  All tiles are 512 wide.
-template <typename T, int WX, int WY, int VXT, bool FULL_OK = true>
+template <typename T, int WX, int WY, int VXT, bool TM, bool FULL_OK = true>
 static KernelFn pick_t2_shape(bool aligned, bool full) {
     if constexpr (FULL_OK) {
-        if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true>;
+        if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true, TM>;
     }
-    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, false>
-                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, false, false>;
+    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, false, TM>
+                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, false, false, TM>;
 }
-template <typename T>
+template <typename T, bool TM>
 static KernelFn pick_t2_kernel(bool aligned, bool full, int block) {
     if constexpr (sizeof(T) == 8) {
-        if (block == 512) return pick_t2_shape<T, 2, 4, 4>(aligned, full);
+        if (block == 512) return pick_t2_shape<T, 2, 4, 4, TM>(aligned, full);
         // the full-width (nx = 512) build of this shape spills to scratch; the general
         // aligned build handles nx = 512 as well
-        return pick_t2_shape<T, 4, 4, 2, false>(aligned, full);
+        return pick_t2_shape<T, 4, 4, 2, TM, false>(aligned, full);
     } else {
-        if (block == 512) return pick_t2_shape<T, 1, 8, 8>(aligned, full);
-        return pick_t2_shape<T, 2, 8, 4>(aligned, full);
+        if (block == 512) return pick_t2_shape<T, 1, 8, 8, TM>(aligned, full);
+        // (the timed full-width build of this shape spills: its timed run uses the general build)
+        return pick_t2_shape<T, 2, 8, 4, TM, !TM>(aligned, full);
     }
 }
 
-template <typename T>
+template <typename T, bool TM>
 static KernelFn pick_kernel(bool is3d, int rows, bool aligned, int block) {
-    return aligned ? pick_kernel_t<T, true>(is3d, rows, block) : pick_kernel_t<T, false>(is3d, rows, block);
+    return aligned ? pick_kernel_t<T, true, TM>(is3d, rows, block) : pick_kernel_t<T, false, TM>(is3d, rows, block);
 }
 
 cfm_status current_device_props(DeviceProps *out) {
@@ -1708,22 +1744,16 @@ using namespace cfm;
 
 struct cfm_jacobi_plan {
     cfm_jacobi_params p{};
-    DeviceProps dev{};
-    size_t elems = 0;
-    size_t elem_size = 0;
+    JacobiConfig c{};
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;  // buffer holding the current field
     GridSync *sync = nullptr;
     double *norm_part = nullptr;
-    int norm_slots = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    KernelFn kernel = nullptr;
-    bool t2 = false;  // more than one sweep per pass (the two-/three-sweep kernels)
-    int sweeps_per_pass = 1;
-    int grid = 0;
-    int chunks = 0, chunk_len = 0, ntx = 0, nty = 0, rows = 0, block = 256;
-    unsigned long long spin_ticks = 0;
+    unsigned long long *timers = nullptr;  // phase_timers: [grid][kTimerSlots] ticks
+    double clock_khz = 100000.0;           // s_memrealtime rate
+    cfm_phase_times phases{};              // breakdown of the last run
 };
 
 static cfm_status validate(const cfm_jacobi_params *p) {
@@ -1741,27 +1771,21 @@ static cfm_status validate(const cfm_jacobi_params *p) {
     return CFM_OK;
 }
 
-extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jacobi_plan **out) {
-    if (!out) return CFM_EINVAL;
-    *out = nullptr;
+cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     cfm_status st = validate(p);
     if (st != CFM_OK) return st;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CFM_EDEVICE;
-    auto *pl = new (std::nothrow) cfm_jacobi_plan();
-    if (!pl) return CFM_ENOMEM;
-    pl->p = *p;
-    st = current_device_props(&pl->dev);
-    if (st != CFM_OK) {
-        delete pl;
-        return st;
-    }
+    st = current_device_props(&c->dev);
+    if (st != CFM_OK) return st;
     const bool is3d = p->nz >= 3;
     const bool f64 = p->prec == CFM_F64;
-    pl->elem_size = f64 ? 8 : 4;
-    pl->elems = (size_t)p->nx * p->ny * (size_t)p->nz;
-    const int vx = 16 / (int)pl->elem_size;
-    const bool aligned = ((size_t)p->nx * pl->elem_size) % 16 == 0;
+    c->elem_size = f64 ? 8 : 4;
+    c->elems = (size_t)p->nx * p->ny * (size_t)p->nz;
+    const int vx = 16 / (int)c->elem_size;
+    const bool aligned = ((size_t)p->nx * c->elem_size) % 16 == 0;
+    // the timed build (phase_timers) of a persistent kernel; one launch per sweep is timed by the caller
+    const bool tm = p->phase_timers != 0 && p->variant != CFM_JACOBI_LAUNCH_PER_ITER;
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
     // two sweeps per pass: persistent Dirichlet; 3D with one tile across x
     const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
@@ -1771,106 +1795,153 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
                         p->tile_rows <= 0 &&
                         (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
     const bool t2_ok = t2_ok3 || t2_ok2;
-    if (p->time_block == 2 && !t2_ok) {
-        delete pl;
+    if (p->time_block == 2 && !t2_ok)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent Dirichlet variant, "
                                      "the default tile shape and, in 3D, nx <= %d", kT2Width);
-    }
-    if (p->time_block >= 3 && !t2_ok2) {
-        delete pl;
+    if (p->time_block >= 3 && !t2_ok2)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: three or four sweeps per pass need the persistent 2D Dirichlet "
                                      "variant and the default tile shape");
-    }
-    if (p->time_block == 4 && p->block_threads > 0 && p->block_threads != 512) {
-        delete pl;
+    if (p->time_block == 4 && p->block_threads > 0 && p->block_threads != 512)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: four sweeps per pass run in 512-thread workgroups");
-    }
-    pl->t2 = t2_ok && p->time_block != 1;
+    c->t2 = t2_ok && p->time_block != 1;
     // 2D: four sweeps per pass by default with the default workgroup (16384^2 FP64 976 GLUP/s vs 790
     // for three and 617 for two; FP32 1612 vs 1523 and 1206); three when 1024 threads are asked for
     const int auto2d = (p->block_threads <= 0 || p->block_threads == 512) ? 4 : 3;
-    pl->sweeps_per_pass =
-        !pl->t2 ? 1 : p->time_block >= 3 ? p->time_block : (p->time_block == 0 && !is3d) ? auto2d : 2;
-    if (pl->t2 && !is3d) {
-        pl->rows = 1;
+    c->sweeps_per_pass =
+        !c->t2 ? 1 : p->time_block >= 3 ? p->time_block : (p->time_block == 0 && !is3d) ? auto2d : 2;
+    if (c->t2 && !is3d) {
+        c->rows = 1;
         // 16384^2: FP64 617 GLUP/s at 1024 threads (2 columns per lane) vs 534 at 512;
         // FP32 1205 at 1024 threads vs 942 at 512
-        const bool wide_ok = !(pl->sweeps_per_pass == 3 && f64 && !aligned);  // (that build spills at 1024)
-        pl->block = p->block_threads > 0 ? p->block_threads : (pl->sweeps_per_pass == 4 || !wide_ok ? 512 : 1024);
-        pl->kernel = pl->sweeps_per_pass == 4   ? pick_t4d_kernel(f64, aligned)
-                     : pl->sweeps_per_pass == 3 ? pick_t3d_kernel(f64, aligned, pl->block)
-                                                : pick_t2d_kernel(f64, aligned, pl->block);
-    } else if (pl->t2) {
-        pl->rows = 1;
+        const bool wide_ok = !(c->sweeps_per_pass == 3 && f64 && !aligned);  // (that build spills at 1024)
+        c->block = p->block_threads > 0 ? p->block_threads : (c->sweeps_per_pass == 4 || !wide_ok ? 512 : 1024);
+        c->kernel = c->sweeps_per_pass == 4 ? (tm ? pick_t4d_kernel<true>(f64, aligned)
+                                                  : pick_t4d_kernel<false>(f64, aligned))
+                    : c->sweeps_per_pass == 3 ? (tm ? pick_t3d_kernel<true>(f64, aligned, c->block)
+                                                    : pick_t3d_kernel<false>(f64, aligned, c->block))
+                                              : (tm ? pick_t2d_kernel<true>(f64, aligned, c->block)
+                                                    : pick_t2d_kernel<false>(f64, aligned, c->block));
+    } else if (c->t2) {
+        c->rows = 1;
         // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428
         // GLUP/s (2 columns, 1024 threads); FP32 one wavefront across x (8 columns per lane)
         // 882 vs 844 (4 columns, 1024 threads)
-        pl->block = p->block_threads > 0 ? p->block_threads : 512;
+        c->block = p->block_threads > 0 ? p->block_threads : 512;
         const bool full = p->nx == kT2Width;
-        pl->kernel = f64 ? pick_t2_kernel<double>(aligned, full, pl->block)
-                         : pick_t2_kernel<float>(aligned, full, pl->block);
+        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(aligned, full, c->block)
+                              : pick_t2_kernel<double, false>(aligned, full, c->block))
+                        : (tm ? pick_t2_kernel<float, true>(aligned, full, c->block)
+                              : pick_t2_kernel<float, false>(aligned, full, c->block));
     } else {
-        pl->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
-        pl->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, pl->rows, f64);
-        if (!is3d && pl->block != 256 && pl->block != default_block(false, pl->rows, f64)) {
-            delete pl;
-            return CFM_EINVAL;
-        }
-        pl->kernel = f64 ? pick_kernel<double>(is3d, pl->rows, aligned, pl->block)
-                         : pick_kernel<float>(is3d, pl->rows, aligned, pl->block);
+        c->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
+        c->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, c->rows, f64);
+        if (!is3d && c->block != 256 && c->block != default_block(false, c->rows, f64))
+            return CFM_FAIL(CFM_EINVAL, "jacobi: 2D %d-row tiles run in 256 or %d-thread workgroups", c->rows,
+                            default_block(false, c->rows, f64));
+        c->kernel = f64 ? (tm ? pick_kernel<double, true>(is3d, c->rows, aligned, c->block)
+                              : pick_kernel<double, false>(is3d, c->rows, aligned, c->block))
+                        : (tm ? pick_kernel<float, true>(is3d, c->rows, aligned, c->block)
+                              : pick_kernel<float, false>(is3d, c->rows, aligned, c->block));
     }
-    if (!pl->kernel) {
-        delete pl;
+    if (!c->kernel)
         return CFM_FAIL(CFM_EINVAL, "jacobi: no kernel for %d-row tiles in %d-thread workgroups (%s, %s rows)",
-                        pl->rows, pl->block, f64 ? "FP64" : "FP32", aligned ? "16-byte aligned" : "unaligned");
-    }
+                        c->rows, c->block, f64 ? "FP64" : "FP32", aligned ? "16-byte aligned" : "unaligned");
     // grid: all co-resident workgroups
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pl->kernel, pl->block, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)c->kernel, c->block, 0) !=
             hipSuccess ||
-        per_cu < 1) {
-        delete pl;
+        per_cu < 1)
         return CFM_EDEVICE;
-    }
     // the barrier keeps one arrival word per workgroup (kMaxGrid of them)
-    const int max_grid = std::min(per_cu * pl->dev.cus, kMaxGrid);
-    pl->grid = p->grid_blocks > 0 ? std::min(p->grid_blocks, max_grid) : max_grid;
+    const int max_grid = std::min(per_cu * c->dev.cus, kMaxGrid);
+    c->grid = p->grid_blocks > 0 ? std::min(p->grid_blocks, max_grid) : max_grid;
     // work units: one per wavefront (per-wave kernels) or per workgroup (3D tile kernel)
-    const bool wg_tile = is3d && pl->block != 256;
-    const int nworkers = wg_tile ? pl->grid : pl->grid * (pl->block / 64);
+    const bool wg_tile = is3d && c->block != 256;
+    const int nworkers = wg_tile ? c->grid : c->grid * (c->block / 64);
     int tile_w = wg_tile ? 64 * vx * 4 : 64 * vx;
-    if (pl->t2 && !is3d) tile_w = 64 * (f64 ? t2d_cols<double>(pl->block) : t2d_cols<float>(pl->block));
-    int tile_h = wg_tile ? pl->rows * (pl->block / 256) : pl->rows;
-    if (pl->t2 && is3d) {
+    if (c->t2 && !is3d) tile_w = 64 * (f64 ? t2d_cols<double>(c->block) : t2d_cols<float>(c->block));
+    int tile_h = wg_tile ? c->rows * (c->block / 256) : c->rows;
+    if (c->t2 && is3d) {
         tile_w = kT2Width;
         tile_h = f64 ? 4 : 8;
     }
-    pl->ntx = (p->nx + tile_w - 1) / tile_w;
+    c->ntx = (p->nx + tile_w - 1) / tile_w;
     const int interior = is3d ? p->nz - 2 : p->ny - 2;
-    pl->nty = is3d ? (p->ny + tile_h - 1) / tile_h : 1;
-    const int tiles = pl->ntx * pl->nty;
+    c->nty = is3d ? (p->ny + tile_h - 1) / tile_h : 1;
+    const int tiles = c->ntx * c->nty;
     int chunks = p->chunks;
     if (chunks <= 0) {
         // 3D tiles of 2 rows: 4 z-chunks per workgroup balance best; else one
         // (two sweeps per pass: one unit per workgroup, long chunks)
-        const int per_worker = (wg_tile && !pl->t2 && pl->rows <= 2) ? 4 : 1;
+        const int per_worker = (wg_tile && !c->t2 && c->rows <= 2) ? 4 : 1;
         chunks = std::max(1, per_worker * nworkers / tiles);
         const int min_len = is3d ? 16 : 64;
         chunks = std::min(chunks, std::max(1, interior / min_len));
     }
     chunks = std::min(chunks, interior);
-    pl->chunk_len = (interior + chunks - 1) / chunks;
-    pl->chunks = (interior + pl->chunk_len - 1) / pl->chunk_len;
-    pl->norm_slots = p->norm_every > 0 ? (p->niter + p->norm_every - 1) / p->norm_every : 0;
-    pl->spin_ticks = spin_ticks(p->spin_limit_us);
+    c->chunk_len = (interior + chunks - 1) / chunks;
+    c->chunks = (interior + c->chunk_len - 1) / c->chunk_len;
+    c->norm_slots = p->norm_every > 0 ? (p->niter + p->norm_every - 1) / p->norm_every : 0;
+    c->spin_ticks = spin_ticks(p->spin_limit_us);
     int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, pl->dev.device) ==
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, c->dev.device) ==
             hipSuccess &&
         rate_khz > 0) {
         const unsigned long long us = p->spin_limit_us > 0 ? p->spin_limit_us : 10000000ull;
-        pl->spin_ticks = us * (unsigned long long)rate_khz / 1000ull;
+        c->spin_ticks = us * (unsigned long long)rate_khz / 1000ull;
     }
+    return CFM_OK;
+}
 
+cfm_status cfm::jacobi_enqueue(const cfm_jacobi_params &p, const JacobiConfig &c, void *b0, void *b1,
+                               GridSync *sync, double *norm_part, unsigned long long *timers, hipStream_t stream,
+                               int *coop) {
+    JacobiArgs A{};
+    A.buf0 = b0;
+    A.buf1 = b1;
+    A.nx = p.nx;
+    A.ny = p.ny;
+    A.nz = p.nz;
+    A.bc = p.bc;
+    A.norm_every = p.norm_every;
+    A.chunks = c.chunks;
+    A.chunk_len = c.chunk_len;
+    A.ntx = c.ntx;
+    A.nty = c.nty;
+    A.norm_part = norm_part;
+    A.sync = sync;
+    A.timers = p.variant == CFM_JACOBI_LAUNCH_PER_ITER ? nullptr : timers;
+    A.no_compute = p.variant == CFM_JACOBI_NO_COMPUTE;
+    *coop = -1;
+    hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, stream, sync, c.spin_ticks, c.grid);
+    CFM_HIP(hipGetLastError());
+    if (p.niter <= 0) return CFM_OK;
+    if (p.variant == CFM_JACOBI_LAUNCH_PER_ITER) {
+        for (int k = 0; k < p.niter; ++k) {
+            A.niter = 1;
+            A.it0 = k;
+            hipLaunchKernelGGL(c.kernel, dim3(c.grid), dim3(c.block), 0, stream, A);
+            CFM_HIP(hipGetLastError());
+        }
+    } else {
+        A.niter = p.niter;
+        A.it0 = 0;
+        void *args[] = {&A};
+        CFM_HIP(launch_persistent((const void *)c.kernel, c.grid, c.block, args, stream, coop));
+    }
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jacobi_plan **out) {
+    if (!out) return CFM_EINVAL;
+    *out = nullptr;
+    JacobiConfig cfg;
+    cfm_status st = jacobi_configure(p, &cfg);
+    if (st != CFM_OK) return st;
+    auto *pl = new (std::nothrow) cfm_jacobi_plan();
+    if (!pl) return CFM_ENOMEM;
+    pl->p = *p;
+    pl->c = cfg;
     auto fail = [&](cfm_status s) {
         cfm_jacobi_plan_destroy(pl);
         return s;
@@ -1879,19 +1950,27 @@ extern "C" cfm_status cfm_jacobi_plan_create(const cfm_jacobi_params *p, cfm_jac
     if (hipEventCreate(&pl->ev0) != hipSuccess || hipEventCreate(&pl->ev1) != hipSuccess)
         return fail(CFM_EDEVICE);
     for (int i = 0; i < 2; ++i)
-        if (hipMalloc(&pl->buf[i], pl->elems * pl->elem_size) != hipSuccess)
+        if (hipMalloc(&pl->buf[i], pl->c.elems * pl->c.elem_size) != hipSuccess)
             return fail(CFM_ENOMEM);
     if (hipMalloc((void **)&pl->sync, sizeof(GridSync)) != hipSuccess) return fail(CFM_ENOMEM);
-    if (pl->norm_slots > 0 &&
-        hipMalloc((void **)&pl->norm_part, sizeof(double) * pl->norm_slots * pl->grid) != hipSuccess)
+    if (pl->c.norm_slots > 0 &&
+        hipMalloc((void **)&pl->norm_part, sizeof(double) * pl->c.norm_slots * pl->c.grid) != hipSuccess)
         return fail(CFM_ENOMEM);
+    if (p->phase_timers && p->variant != CFM_JACOBI_LAUNCH_PER_ITER) {
+        if (hipMalloc((void **)&pl->timers, sizeof(unsigned long long) * kTimerSlots * pl->c.grid) != hipSuccess)
+            return fail(CFM_ENOMEM);
+        int rate_khz = 0;
+        if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, pl->c.dev.device) == hipSuccess &&
+            rate_khz > 0)
+            pl->clock_khz = rate_khz;
+    }
     *out = pl;
     return CFM_OK;
 }
 
 extern "C" cfm_status cfm_jacobi_plan_load(cfm_jacobi_plan *pl, const void *init_host) {
     if (!pl || !init_host) return CFM_EINVAL;
-    const size_t bytes = pl->elems * pl->elem_size;
+    const size_t bytes = pl->c.elems * pl->c.elem_size;
     CFM_HIP(hipMemcpyAsync(pl->buf[0], init_host, bytes, hipMemcpyHostToDevice, pl->stream));
     CFM_HIP(hipMemcpyAsync(pl->buf[1], pl->buf[0], bytes, hipMemcpyDeviceToDevice, pl->stream));
     CFM_HIP(hipStreamSynchronize(pl->stream));
@@ -1899,45 +1978,44 @@ extern "C" cfm_status cfm_jacobi_plan_load(cfm_jacobi_plan *pl, const void *init
     return CFM_OK;
 }
 
+int cfm::jacobi_passes(const cfm_jacobi_params &p, const JacobiConfig &c) {
+    if (p.variant == CFM_JACOBI_NO_COMPUTE) return 0;
+    return (p.niter + c.sweeps_per_pass - 1) / c.sweeps_per_pass;
+}
+
+void cfm::jacobi_fill_timing(cfm_timing *t, const cfm_jacobi_params &p, const JacobiConfig &c, float ms,
+                             double wall_ms, int coop) {
+    if (!t) return;
+    t->kernel_ms = ms;
+    t->wall_ms = wall_ms;
+    t->grid_blocks = c.grid;
+    t->block_threads = c.block;
+    t->iterations = p.niter;
+    t->outer_iterations = jacobi_passes(p, c);
+    t->cooperative = coop;
+}
+
+static cfm_status jacobi_run(cfm_jacobi_plan *pl, double *norms, int norm_cap, int *norm_count, cfm_timing *t);
+
+// A roctx range around every run (a no-op unless a profiler listens).
 extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, int norm_cap,
                                           int *norm_count, cfm_timing *t) {
     if (!pl) return CFM_EINVAL;
+    roctxRangePush(pl->p.nz >= 3 ? "cfm_jacobi_plan_run 3D" : "cfm_jacobi_plan_run 2D");
+    const cfm_status st = jacobi_run(pl, norms, norm_cap, norm_count, t);
+    roctxRangePop();
+    return st;
+}
+
+static cfm_status jacobi_run(cfm_jacobi_plan *pl, double *norms, int norm_cap, int *norm_count, cfm_timing *t) {
     const auto w0 = std::chrono::steady_clock::now();
     const cfm_jacobi_params &p = pl->p;
-    JacobiArgs A{};
-    A.buf0 = pl->buf[pl->cur];
-    A.buf1 = pl->buf[pl->cur ^ 1];
-    A.nx = p.nx;
-    A.ny = p.ny;
-    A.nz = p.nz;
-    A.bc = p.bc;
-    A.norm_every = p.norm_every;
-    A.chunks = pl->chunks;
-    A.chunk_len = pl->chunk_len;
-    A.ntx = pl->ntx;
-    A.nty = pl->nty;
-    A.norm_part = pl->norm_part;
-    A.sync = pl->sync;
-    A.no_compute = p.variant == CFM_JACOBI_NO_COMPUTE;
-    hipLaunchKernelGGL(reset_sync_kernel, dim3(8), dim3(256), 0, pl->stream, pl->sync, pl->spin_ticks, pl->grid);
-    CFM_HIP(hipGetLastError());
-    CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
+    const JacobiConfig &c = pl->c;
     int coop = -1;
-    if (p.niter > 0) {
-        if (p.variant == CFM_JACOBI_LAUNCH_PER_ITER) {
-            for (int k = 0; k < p.niter; ++k) {
-                A.niter = 1;
-                A.it0 = k;
-                hipLaunchKernelGGL(pl->kernel, dim3(pl->grid), dim3(pl->block), 0, pl->stream, A);
-                CFM_HIP(hipGetLastError());
-            }
-        } else {
-            A.niter = p.niter;
-            A.it0 = 0;
-            void *args[] = {&A};
-            CFM_HIP(launch_persistent((const void *)pl->kernel, pl->grid, pl->block, args, pl->stream, &coop));
-        }
-    }
+    CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
+    cfm_status st = jacobi_enqueue(p, c, pl->buf[pl->cur], pl->buf[pl->cur ^ 1], pl->sync, pl->norm_part,
+                                   pl->timers, pl->stream, &coop);
+    if (st != CFM_OK) return st;
     CFM_HIP(hipEventRecord(pl->ev1, pl->stream));
     CFM_HIP(hipEventSynchronize(pl->ev1));
     float ms = 0.f;
@@ -1945,37 +2023,41 @@ extern "C" cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *pl, double *norms, in
     GridSync hs{};
     CFM_HIP(hipMemcpy(&hs, pl->sync, sizeof(hs), hipMemcpyDeviceToHost));
     if (hs.abort) return barrier_timeout("jacobi", coop);
+    pl->phases = cfm_phase_times{};
+    if (pl->timers && p.niter > 0) {
+        std::vector<unsigned long long> ticks((size_t)kTimerSlots * c.grid);
+        CFM_HIP(hipMemcpy(ticks.data(), pl->timers, sizeof(unsigned long long) * ticks.size(), hipMemcpyDeviceToHost));
+        summarize_phases(ticks.data(), c.grid, pl->clock_khz, &pl->phases);
+    }
     int produced = 0;
-    if (pl->norm_slots > 0 && p.variant != CFM_JACOBI_NO_COMPUTE) {
-        std::vector<double> part((size_t)pl->norm_slots * pl->grid);
-        CFM_HIP(hipMemcpy(part.data(), pl->norm_part, sizeof(double) * part.size(),
-                          hipMemcpyDeviceToHost));
-        for (int s = 0; s < pl->norm_slots; ++s) {
+    if (c.norm_slots > 0 && p.variant != CFM_JACOBI_NO_COMPUTE) {
+        std::vector<double> part((size_t)c.norm_slots * c.grid);
+        CFM_HIP(hipMemcpy(part.data(), pl->norm_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+        for (int s = 0; s < c.norm_slots; ++s) {
             double acc = 0.0;
-            for (int b = 0; b < pl->grid; ++b) acc += part[(size_t)s * pl->grid + b];
+            for (int b = 0; b < c.grid; ++b) acc += part[(size_t)s * c.grid + b];
             if (norms && s < norm_cap) norms[s] = std::sqrt(acc);
         }
-        produced = pl->norm_slots;
+        produced = c.norm_slots;
     }
     if (norm_count) *norm_count = produced;
-    const int spp = pl->sweeps_per_pass;
-    const int passes = (p.niter + spp - 1) / spp;
-    if (p.variant != CFM_JACOBI_NO_COMPUTE) pl->cur ^= passes & 1;
-    if (t) {
-        t->kernel_ms = ms;
-        t->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
-        t->grid_blocks = pl->grid;
-        t->block_threads = pl->block;
-        t->iterations = p.niter;
-        t->outer_iterations = p.variant == CFM_JACOBI_NO_COMPUTE ? 0 : passes;
-        t->cooperative = coop;
-    }
+    pl->cur ^= jacobi_passes(p, c) & 1;
+    jacobi_fill_timing(t, p, c, ms,
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count(), coop);
+    return CFM_OK;
+}
+
+extern "C" cfm_status cfm_jacobi_plan_phase_times(cfm_jacobi_plan *pl, cfm_phase_times *out) {
+    if (!pl || !out) return CFM_EINVAL;
+    if (!pl->timers)
+        return CFM_FAIL(CFM_EINVAL, "jacobi: the plan was created without phase_timers (or runs one launch per sweep)");
+    *out = pl->phases;
     return CFM_OK;
 }
 
 extern "C" cfm_status cfm_jacobi_plan_download(cfm_jacobi_plan *pl, void *out_host) {
     if (!pl || !out_host) return CFM_EINVAL;
-    CFM_HIP(hipMemcpy(out_host, pl->buf[pl->cur], pl->elems * pl->elem_size, hipMemcpyDeviceToHost));
+    CFM_HIP(hipMemcpy(out_host, pl->buf[pl->cur], pl->c.elems * pl->c.elem_size, hipMemcpyDeviceToHost));
     return CFM_OK;
 }
 
@@ -1992,6 +2074,7 @@ extern "C" void cfm_jacobi_plan_destroy(cfm_jacobi_plan *pl) {
         if (pl->buf[i]) (void)hipFree(pl->buf[i]);
     if (pl->sync) (void)hipFree(pl->sync);
     if (pl->norm_part) (void)hipFree(pl->norm_part);
+    if (pl->timers) (void)hipFree(pl->timers);
     if (pl->ev0) (void)hipEventDestroy(pl->ev0);
     if (pl->ev1) (void)hipEventDestroy(pl->ev1);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);

@@ -82,6 +82,9 @@ typedef struct {
                          (2D: 4, or 3 with 1024-thread workgroups; 3D: 2 when nx <= 512; with the
                          default tile shape, else 1), 1, 2, or 3 and 4 (2D only; 4 in 512-thread
                          workgroups) */
+    int phase_timers; /* nonzero: every workgroup of the persistent kernel records the time it spends
+                         sweeping and waiting at the device-wide barrier (cfm_jacobi_plan_phase_times);
+                         results are unchanged */
 } cfm_jacobi_params;
 
 typedef struct {
@@ -96,6 +99,30 @@ typedef struct {
                            cooperative grant (co-residency expected, not guaranteed; the barrier
                            watchdog bounds every wait); -1 = no persistent launch */
 } cfm_timing;
+
+/* In-place time breakdown of a run (phase_timers set), per workgroup of the
+ * persistent kernel, from its own clock (s_memrealtime) at every phase
+ * boundary; the phases of one workgroup add up to its time in the kernel.  No
+ * counterpart in the reference, whose breakdown exists only for its
+ * host-driven versions (NVTX ranges per operation,
+ * CG/src/profiling/discrete-standard.cu:222-264, summed by
+ * CG/scripts/measure_operation_breakdown.py). */
+enum {
+    CFM_PHASE_SPMV = 0,    /* CG: SpMV with its fused dot partials (Pipelined: the whole fused
+                              iteration, SpMV + two dots + six updates); Jacobi: the sweeps of a pass */
+    CFM_PHASE_UPDATE = 1,  /* CG: vector updates (axpy) with their fused dot partials */
+    CFM_PHASE_REDUCE = 2,  /* CG: every workgroup's reduction of the chunk partials to the scalars */
+    CFM_PHASE_BARRIER = 3, /* waiting at the device-wide barrier (the slowest workgroup + the barrier) */
+    CFM_PHASE_OTHER = 4,   /* set-up: initial residual, refinement outer steps */
+    CFM_NPHASES = 5
+};
+typedef struct {
+    double ms[CFM_NPHASES];     /* per phase: mean over workgroups */
+    double max_ms[CFM_NPHASES]; /* per phase: the largest workgroup */
+    double total_ms;            /* mean over workgroups of the first-to-last clock reading */
+    int workgroups;             /* workgroups averaged (0: no breakdown recorded) */
+    int host_timed;             /* 1: discrete schedule timed per operation on the host */
+} cfm_phase_times;
 
 typedef struct cfm_jacobi_plan cfm_jacobi_plan;
 typedef struct cfm_cg_plan cfm_cg_plan;
@@ -131,6 +158,8 @@ cfm_status cfm_jacobi_plan_load(cfm_jacobi_plan *plan, const void *init_host);
 cfm_status cfm_jacobi_plan_run(cfm_jacobi_plan *plan, double *norms, int norm_cap,
                                int *norm_count, cfm_timing *t);
 cfm_status cfm_jacobi_plan_download(cfm_jacobi_plan *plan, void *out_host);
+/* Breakdown of the last run of a plan created with phase_timers set. */
+cfm_status cfm_jacobi_plan_phase_times(cfm_jacobi_plan *plan, cfm_phase_times *out);
 /* Borrowed device pointer to the current field (valid until the next run). */
 cfm_status cfm_jacobi_plan_field(cfm_jacobi_plan *plan, void **device_ptr);
 void cfm_jacobi_plan_destroy(cfm_jacobi_plan *plan);
@@ -165,6 +194,13 @@ typedef struct {
     int matrix_format;   /* 0 = auto (single-system Pipelined with >= 2^20 rows: DIA when the CSR
                             is banded with <= 16 diagonals and sorted rows; otherwise SELL-64),
                             1 = SELL-64, 2 = DIA (every system of a batch; ENOTSUP if not banded) */
+    int phase_timers;    /* nonzero: in-place time breakdown of the solve (cfm_cg_plan_phase_times):
+                            the persistent kernel's workgroups record a clock at every phase boundary
+                            (a separate kernel build: the default build has no timer code); the
+                            discrete schedule times each operation on the host, synchronising after
+                            it, and marks it with a roctx range, as the reference's profiling
+                            versions do (CG/src/profiling/discrete-standard.cu:222-264).  Results are
+                            unchanged (same bits). */
 } cfm_cg_params;
 
 /* Replaces SingleStreamStandard::init / SingleStreamPipelined::init and
@@ -185,6 +221,8 @@ cfm_status cfm_cg_plan_set_x0(cfm_cg_plan *plan, const double *x0_host);
 cfm_status cfm_cg_plan_solve(cfm_cg_plan *plan, double *hist, int hist_cap, int *iters,
                              cfm_timing *t);
 cfm_status cfm_cg_plan_get_x(cfm_cg_plan *plan, double *x_host);
+/* Breakdown of the last solve of a plan created with phase_timers set. */
+cfm_status cfm_cg_plan_phase_times(cfm_cg_plan *plan, cfm_phase_times *out);
 void cfm_cg_plan_destroy(cfm_cg_plan *plan);
 /* One shot on host arrays. x_io: x0 in, solution out. */
 cfm_status cfm_cg_solve(int n, int nnz, const int *rowptr, const int *col, const double *val,

@@ -109,7 +109,7 @@ def _write_mm_symmetric(path, rp, col, val):
         f.write("%%MatrixMarket matrix coordinate real symmetric\n% 3D Poisson, lower triangle\n")
         f.write(f"{n} {n} {r.size}\n")
         for k in order:
-            f.write(f"{r[k]} {c[k]} {v[k]!r}\n")
+            f.write(f"{r[k]} {c[k]} {float(v[k])!r}\n")
 
 
 def test_matrix_market_file_equals_oracle(gpu, orc, tmp_path):

@@ -41,7 +41,7 @@ EXPORTS = (
     "cfm_unet_get_params", "cfm_unet_forward", "cfm_unet_gradient", "cfm_unet_train_step",
     "cfm_cg_batch_x0_rejected", "cfm_cg_batch_load_features", "cfm_cg_batch_warm_start", "cfm_gen_varcoef3d_coef",
     "cfm_cg_batch_smooth_x0", "cfm_gen_tridiag", "cfm_mm_read", "cfm_host_free", "cfm_jacobi_plan_phase_times",
-    "cfm_cg_plan_phase_times",
+    "cfm_cg_plan_phase_times", "cfm_jacobi_run_noalloc",
 )
 PHASES = ("spmv", "update", "reduce", "barrier", "other")
 
@@ -132,6 +132,8 @@ def _load():
     L.cfm_jacobi_run.argtypes = [ctypes.POINTER(JacobiParams), _vp, _vp, _dp, ctypes.c_int, _ip,
                                  ctypes.POINTER(Timing)]
     L.cfm_jacobi_init_field.argtypes = [ctypes.c_int] * 5 + [_vp]
+    L.cfm_jacobi_run_noalloc.argtypes = [ctypes.POINTER(JacobiParams), _vp, _vp, _vp, _dp, ctypes.c_int, _ip,
+                                         ctypes.POINTER(Timing)]
     L.cfm_cg_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(CgParams), ctypes.POINTER(_vp)]
     L.cfm_cg_plan_set_rhs.argtypes = [_vp, _dp]
@@ -395,6 +397,22 @@ def jacobi(init, niter, bc=BC_DIRICHLET, norm_every=0, variant=PERSISTENT, **tun
         norms, t = plan.run()
         out = plan.download()
     return out, norms, t.as_dict()
+
+
+def jacobi_noalloc(params, field, work, stream=None):
+    """cfm_jacobi_run_noalloc on two device buffers (DeviceBuffer or raw
+    device addresses) of params' field size: field holds the initial field
+    and receives the result, work is overwritten.  Returns (norms, timing_dict)."""
+    p = params
+    cap = (p.niter + p.norm_every - 1) // p.norm_every if p.norm_every > 0 else 0
+    norms = np.zeros(max(cap, 1))
+    cnt = ctypes.c_int(0)
+    t = Timing()
+    fptr = field.ptr if isinstance(field, DeviceBuffer) else _vp(field)
+    wptr = work.ptr if isinstance(work, DeviceBuffer) else _vp(work)
+    _check(lib.cfm_jacobi_run_noalloc(ctypes.byref(p), fptr, wptr, _vp(stream), norms.ctypes.data_as(_dp), cap,
+                                      ctypes.byref(cnt), ctypes.byref(t)), "cfm_jacobi_run_noalloc")
+    return norms[: cnt.value], t.as_dict()
 
 
 # ---------------------------------------------------------------------- CG --

@@ -21,7 +21,7 @@ import tempfile
 PKG = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["api.hip", "jacobi.hip", "cg.hip", "cg_batch.hip", "unet.hip"]
+SOURCES = ["api.hip", "jacobi.hip", "jacobi_noalloc.hip", "cg.hip", "cg_batch.hip", "unet.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-I" + os.path.join(REPO, "include")]
 

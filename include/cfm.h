@@ -166,6 +166,21 @@ void cfm_jacobi_plan_destroy(cfm_jacobi_plan *plan);
 /* One shot: create, load, run, download, destroy. */
 cfm_status cfm_jacobi_run(const cfm_jacobi_params *p, const void *init_host, void *out_host,
                           double *norms, int norm_cap, int *norm_count, cfm_timing *t);
+/* No counterpart in the reference: cfm_jacobi_run for callers that cannot
+ * allocate (tight loops, harnesses that forbid hipMalloc).  It makes no
+ * hipMalloc / hipFree and creates no stream or event: the barrier state and
+ * the update-norm partials live in a 2 MiB device array of the library, one
+ * per GPU, held until the process ends.  field (device, borrowed): nz*ny*nx
+ * elements of the plan's type, the initial field in, the result out; work
+ * (device, borrowed, same size, distinct): overwritten (it starts as a copy
+ * of field).  stream: a hipStream_t, NULL for the null stream; the call
+ * returns when the run is done.  Same kernels and bits as the plan path.
+ * Refused with CFM_EINVAL: phase_timers, and norm slots x (workgroups + 1)
+ * beyond the array's 257,888 FP64 words (e.g. 1003 norms at 256 workgroups).
+ * Calls that overlap on one GPU share the array and race.  kernel_ms is the
+ * host-timed run (no events). */
+cfm_status cfm_jacobi_run_noalloc(const cfm_jacobi_params *p, void *field, void *work, void *stream,
+                                  double *norms, int norm_cap, int *norm_count, cfm_timing *t);
 /* Host generator of the built-in initial fields (prec selects double/float output).
  * Replaces initialize_boundaries (Stencil/jacobi2D/src/common.cu:15-26,
  * Stencil/jacobi3D/src/common.cu:15-29). */

@@ -637,7 +637,17 @@ struct T2Tile {
 // vector loads and stores; the two boundary columns are rewritten with their
 // own, unchanged value).  Wave roles (top / middle / bottom of the tile) are
 // wave-uniform branches, so no per-lane selects are spent on them.
-template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, int CLA = 1>
+//
+// PER: the reference's periodic z handling (after every sweep the last
+// interior plane is copied to plane 0 and the first to plane nz-1).  Level 0
+// is read as stored (its planes 0 and nz-1 hold the previous pass's copies,
+// or the initial field); level 1 on plane 0 is therefore level 1 on plane
+// nz-2, and on plane nz-1 it is level 1 on plane 1: the chunk starting at
+// plane 1 computes the former in a prologue from stored planes nz-3..nz-1,
+// the chunk ending at plane nz-2 the latter from stored planes 0..2 in its
+// last step.  Whoever writes level 2 (or, on a one-sweep pass, level 1) of
+// planes nz-2 and 1 also writes the whole row into plane 0 / nz-1.
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, int CLA = 1>
 __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
                                            int ty, int zc, bool single, bool want0, bool want1, double &n0,
                                            double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
@@ -744,20 +754,28 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
         }
     };
 
-    const int p0 = single ? zb : zb - 1;
-    const int p1 = single ? ze - 1 : ze;  // inclusive
+    // the whole row (boundary columns included) of a periodic halo plane
+    auto store_copy = [&](int z, const T (&out)[VX]) {
+        if (y >= ny) return;  // wave-uniform
+        T *drow = dst + (size_t)z * plane + (size_t)y * nx;
+        if (FULL) {
+            constexpr int E = 16 / sizeof(T);
 #pragma unroll
-    for (int i = 0; i < NC - 1; ++i) ld_col(p0 - 1 + i, col[i]);
-    ld_side(p0, sid[0]);
-    put_l0e(lds[p0 % 3], col[1]);
-    __syncthreads();
-    for (int p = p0; p <= p1; ++p) {
-        ld_col(p + 1 + CLA, col[NC - 1]);  // consumed at step p+CLA
-        ld_side(p + 1, sid[1]);  // consumed at step p+1
-        typename Tl::Buf &Bp = lds[p % 3];
-        const bool pin = p >= 1 && p <= nz - 2;  // interior plane
-        const bool nrm = want0 && p >= zb && p < ze;
-        // ---- level 1 on plane p: own row, and the tile-halo row of the top/bottom wave
+            for (int q = 0; q < VX / E; ++q) {
+                typename Vec16<T>::type t;
+#pragma unroll
+                for (int e = 0; e < E; ++e) t[e] = out[q * E + e];
+                store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E), t);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k)
+                if (x + k < nx) drow[x + k] = out[k];
+        }
+    };
+    // level 1 on the plane whose level-0 rows are in col[0..2] / sid[0]: own row,
+    // and the tile-halo row of the top/bottom wave
+    auto level1 = [&](const typename Tl::Buf &Bp, bool pin, bool nrm) {
         if (top) {
             l1row(Bp, 0, y, pin, col[1][0], sid[0][1], col[1][1], col[2][0], col[0][0], l1o[2], nrm);
             l1row(Bp, 1, y - 1, pin, col[1][1], col[1][0], sid[0][0], col[2][1], col[0][1], l1h[1], false);
@@ -767,10 +785,57 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
         } else {
             l1row(Bp, 0, y, pin, col[1][0], sid[0][1], sid[0][0], col[2][0], col[0][0], l1o[2], nrm);
         }
+    };
+
+    int p0 = single ? zb : zb - 1;
+    const int p1 = single ? ze - 1 : ze;  // inclusive
+    if (PER && !single && zb == 1) {
+        // level 1 on plane 0 = level 1 on plane nz-2, from stored planes nz-3 .. nz-1
+        ld_col(nz - 3, col[0]);
+        ld_col(nz - 2, col[1]);
+        ld_col(nz - 1, col[2]);
+        ld_side(nz - 2, sid[0]);
+        put_l0e(lds[0], col[1]);
+        __syncthreads();
+        level1(lds[0], true, false);
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+            l1o[1][k] = l1o[2][k];
+            l1h[0][k] = l1h[1][k];
+        }
+        p0 = 1;  // the stream starts at plane 1 (its level-0 edges go to lds[1], not lds[0])
+    }
+#pragma unroll
+    for (int i = 0; i < NC - 1; ++i) ld_col(p0 - 1 + i, col[i]);
+    ld_side(p0, sid[0]);
+    put_l0e(lds[p0 % 3], col[1]);
+    __syncthreads();
+    for (int p = p0; p <= p1; ++p) {
+        ld_col(p + 1 + CLA, col[NC - 1]);  // consumed at step p+CLA
+        ld_side(p + 1, sid[1]);  // consumed at step p+1
+        typename Tl::Buf &Bp = lds[p % 3];
+        bool pin = p >= 1 && p <= nz - 2;  // interior plane
+        if (PER && !single && p == nz - 1) {
+            // level 1 on plane nz-1 = level 1 on plane 1, from stored planes 0 .. 2 (nobody
+            // reads lds[p % 3] between the previous step's barrier and the one below)
+            ld_col(0, col[0]);
+            ld_col(1, col[1]);
+            ld_col(2, col[2]);
+            ld_side(1, sid[0]);
+            put_l0e(Bp, col[1]);
+            __syncthreads();
+            pin = true;
+        }
+        const bool nrm = want0 && p >= zb && p < ze;
+        level1(Bp, pin, nrm);
         typename Tl::Buf &Bn = lds[(p + 1) % 3];
         if (single) {
             // one sweep only: store the own level-1 row of plane p
             if (y >= 1 && y <= ny - 2) store_row(p, l1o[2], xin);
+            if (PER) {  // l1o[2] holds the stored value wherever the point is not interior
+                if (p == nz - 2) store_copy(0, l1o[2]);
+                if (p == 1) store_copy(nz - 1, l1o[2]);
+            }
             put_l0e(Bn, col[2]);
             __syncthreads();
         } else {
@@ -815,6 +880,15 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
                     }
                 }
                 store_row(zo, out, xin);
+                if (PER) {
+                    if (zo == nz - 2) store_copy(0, out);
+                    if (zo == 1) store_copy(nz - 1, out);
+                }
+            } else if (PER && zo >= zb && (zo == 1 || zo == nz - 2) && (y == 0 || y == ny - 1)) {
+                // boundary rows of a copied plane keep their stored values, which level 1 of
+                // plane zo (l1o[1]) carries there (col[0] may hold the reloaded plane 0 here)
+                if (zo == nz - 2) store_copy(0, l1o[1]);
+                if (zo == 1) store_copy(nz - 1, l1o[1]);
             }
         }
         // ---- slide the windows
@@ -834,7 +908,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     __syncthreads();  // the LDS buffers are reused by the next unit
 }
 
-template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool TIMED>
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
     using Tl = T2Tile<T, WX, WY, VXT>;
     __shared__ typename Tl::Buf bufs[3];
@@ -858,7 +932,8 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
         for (int u = lb; u < nunits; u += g) {
             const int zc = u / A.nty;
             const int ty = u - zc * A.nty;
-            sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1, bufs);
+            sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1,
+                                                           bufs);
         }
         if (want0) {
             const double s = block_sum_waves<Tl::BLOCK / 64>(n0, red);
@@ -1699,23 +1774,30 @@ constexpr int kT2Width = 512;
 // threads) or, with 512 threads, 2 x 4 wavefronts of 4 columns per lane
 // (twice the registers per thread, half the per-wave overhead per point).
 // FP32: 2 x 8 wavefronts of 4 columns per lane.  All tiles are 512 wide.
-template <typename T, int WX, int WY, int VXT, bool TM, bool FULL_OK = true>
+template <typename T, int WX, int WY, int VXT, bool TM, bool FULL_OK = true, bool PER = false>
 static KernelFn pick_t2_shape(bool aligned, bool full) {
     if constexpr (FULL_OK) {
-        if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true, TM>;
+        if (full) return (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, true, PER, TM>;
     }
-    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, false, TM>
-                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, false, false, TM>;
+    return aligned ? (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, true, false, PER, TM>
+                   : (KernelFn)jacobi3d_t2_kernel<T, WX, WY, VXT, false, false, PER, TM>;
 }
+// periodic z: built for the default 512-thread shapes only (null otherwise)
 template <typename T, bool TM>
-static KernelFn pick_t2_kernel(bool aligned, bool full, int block) {
+static KernelFn pick_t2_kernel(bool aligned, bool full, int block, bool periodic) {
     if constexpr (sizeof(T) == 8) {
-        if (block == 512) return pick_t2_shape<T, 2, 4, 4, TM>(aligned, full);
+        if (block == 512)
+            return periodic ? pick_t2_shape<T, 2, 4, 4, TM, true, true>(aligned, full)
+                            : pick_t2_shape<T, 2, 4, 4, TM>(aligned, full);
+        if (periodic) return nullptr;
         // the full-width (nx = 512) build of this shape spills to scratch; the general
         // aligned build handles nx = 512 as well
         return pick_t2_shape<T, 4, 4, 2, TM, false>(aligned, full);
     } else {
-        if (block == 512) return pick_t2_shape<T, 1, 8, 8, TM>(aligned, full);
+        if (block == 512)
+            return periodic ? pick_t2_shape<T, 1, 8, 8, TM, true, true>(aligned, full)
+                            : pick_t2_shape<T, 1, 8, 8, TM>(aligned, full);
+        if (periodic) return nullptr;
         // (the timed full-width build of this shape spills: its timed run uses the general build)
         return pick_t2_shape<T, 2, 8, 4, TM, !TM>(aligned, full);
     }
@@ -1788,16 +1870,19 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     const bool tm = p->phase_timers != 0 && p->variant != CFM_JACOBI_LAUNCH_PER_ITER;
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
     // two sweeps per pass: persistent Dirichlet; 3D with one tile across x
-    const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
-                        p->nx <= kT2Width && p->tile_rows <= 0 &&
-                        (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
+    // (3D periodic: 512-thread workgroups only)
+    const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->nx <= kT2Width && p->tile_rows <= 0 &&
+                        (p->bc == CFM_BC_DIRICHLET
+                             ? (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512)
+                             : (p->block_threads <= 0 || p->block_threads == 512));
     const bool t2_ok2 = !is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
                         p->tile_rows <= 0 &&
                         (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
     const bool t2_ok = t2_ok3 || t2_ok2;
     if (p->time_block == 2 && !t2_ok)
-        return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent Dirichlet variant, "
-                                     "the default tile shape and, in 3D, nx <= %d", kT2Width);
+        return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent variant, the default tile "
+                                     "shape and, in 2D, Dirichlet rows, in 3D nx <= %d (periodic z: 512-thread "
+                                     "workgroups)", kT2Width);
     if (p->time_block >= 3 && !t2_ok2)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: three or four sweeps per pass need the persistent 2D Dirichlet "
                                      "variant and the default tile shape");
@@ -1828,10 +1913,11 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
         // 882 vs 844 (4 columns, 1024 threads)
         c->block = p->block_threads > 0 ? p->block_threads : 512;
         const bool full = p->nx == kT2Width;
-        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(aligned, full, c->block)
-                              : pick_t2_kernel<double, false>(aligned, full, c->block))
-                        : (tm ? pick_t2_kernel<float, true>(aligned, full, c->block)
-                              : pick_t2_kernel<float, false>(aligned, full, c->block));
+        const bool per = p->bc == CFM_BC_PERIODIC;
+        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(aligned, full, c->block, per)
+                              : pick_t2_kernel<double, false>(aligned, full, c->block, per))
+                        : (tm ? pick_t2_kernel<float, true>(aligned, full, c->block, per)
+                              : pick_t2_kernel<float, false>(aligned, full, c->block, per));
     } else {
         c->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
         c->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, c->rows, f64);

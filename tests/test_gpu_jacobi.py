@@ -224,6 +224,41 @@ def test_two_sweeps_per_pass_norms_and_units(gpu, orc, norm_every, chunks, grid,
            block_threads=block)
 
 
+@pytest.mark.parametrize("dtype,block", [(np.float64, 0), (np.float64, 512), (np.float32, 0)])
+@pytest.mark.parametrize("shape", T2_SHAPES)
+@pytest.mark.parametrize("niter", [1, 2, 3, 8])
+def test_two_sweeps_per_pass_periodic_bitexact(gpu, orc, dtype, block, shape, niter):
+    """Temporal blocking with the reference's periodic z halos (planes 0 and
+    nz-1 rewritten from nz-2 and 1 after every sweep): level 1 on the halo
+    planes comes from the opposite end of the field; the field (halo planes
+    included) and the norms equal single sweeps bit for bit."""
+    init = _rand(shape, dtype, 21 + niter)
+    t = _check(gpu, orc, init, niter, 0, norm_every=3, time_block=2, block_threads=block)
+    assert t["outer_iterations"] == (niter + 1) // 2
+
+
+@pytest.mark.parametrize("norm_every", [1, 3])
+@pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (5, 0), (0, 1), (3, 7), (28, 0)])
+def test_two_sweeps_per_pass_periodic_units(gpu, orc, norm_every, chunks, grid):
+    """Periodic halos across every chunking: one chunk holding both ends,
+    chunks of one plane, one workgroup walking every unit."""
+    init = _rand((30, 45, 130), np.float64, 6)
+    _check(gpu, orc, init, 9, 0, norm_every=norm_every, time_block=2, chunks=chunks, grid_blocks=grid)
+
+
+def test_reference_mode_3d_full_size_bitexact(gpu, orc):
+    """The jacobi3D CLI's default mode at the benchmark size: 512^3 FP64, sin
+    x faces, periodic z halos, 1000 sweeps (two per HBM pass), update norm
+    every 100 -- field bit for bit against the oracle's host loop."""
+    init = gpu.init_field(512, 512, 512)
+    out, norms, t = gpu.jacobi(init, 1000, bc=gpu.BC_PERIODIC, norm_every=100)
+    assert t["outer_iterations"] == 500
+    ref, rnorms = orc.jacobi(init, 1000, orc.BC_PERIODIC, 100)
+    del init
+    assert np.array_equal(out, ref)
+    assert np.allclose(norms, rnorms, rtol=NORM_RTOL, atol=0.0)
+
+
 def test_two_sweeps_per_pass_matches_one_sweep_per_pass(gpu):
     init = _rand((20, 31, 512), np.float64, 8)
     a, na, ta = gpu.jacobi(init, 10, bc=1, norm_every=5, time_block=1)
@@ -311,7 +346,7 @@ def test_two_sweeps_per_pass_matches_one_sweep_per_pass_2d(gpu):
     assert (ta["outer_iterations"], tb["outer_iterations"]) == (11, 6)
 
 
-@pytest.mark.parametrize("kw", [dict(bc=0), dict(bc=1, nx=600), dict(bc=1, variant=0), dict(bc=1, tile_rows=2),
+@pytest.mark.parametrize("kw", [dict(bc=0, block_threads=1024), dict(bc=1, nx=600), dict(bc=1, variant=0), dict(bc=1, tile_rows=2),
                                 dict(bc=0, nz=1), dict(bc=1, nz=1, tile_rows=8), dict(bc=1, nz=1, variant=0),
                                 dict(bc=1, nz=1, block_threads=256)])
 def test_two_sweeps_per_pass_rejects_unsupported(gpu, kw):

@@ -8,7 +8,9 @@
 // pass, "PERKS" the persistent kernel that keeps the intermediate sweeps on
 // chip (two sweeps per HBM pass in 3D, four in 2D, where the shape allows
 // it; -tblock 1|2|3|4 to force), and the "(No computation)" versions run the
-// persistent kernel's barriers only.
+// persistent kernel's barriers only.  -phase_timers prints the persistent
+// kernel's in-place breakdown (time sweeping vs waiting at the device-wide
+// barrier, per workgroup).
 //
 // -compare runs the launch-per-sweep scheme first (the reference's
 // single_gpu, Stencil/jacobi2D/src/common.cu:154-273) and compares every
@@ -63,6 +65,8 @@ inline int run_jacobi(int argc, char **argv, int dims) {
     p.block_threads = get_argval<int>(b, e, "-block", 0);
     p.time_block = get_argval<int>(b, e, "-tblock", selection == 5 ? 0 : 1);
     const bool json = get_arg(b, e, "-json");
+    // -phase_timers: the persistent kernel's in-place breakdown (sweeps / barrier wait per workgroup)
+    p.phase_timers = get_arg(b, e, "-phase_timers") ? 1 : 0;
     p.prec = prec == "f32" ? CFM_F32 : CFM_F64;
     p.bc = bc == "dirichlet" ? CFM_BC_DIRICHLET : CFM_BC_PERIODIC;
     p.variant = variants[selection];
@@ -107,6 +111,9 @@ inline int run_jacobi(int argc, char **argv, int dims) {
     st = cfm_jacobi_plan_load(plan, field.data());
     if (st == CFM_OK) st = cfm_jacobi_plan_run(plan, norms.data(), ncap, &nnorm, &t);
     if (st == CFM_OK) st = cfm_jacobi_plan_download(plan, out.data());
+    cfm_phase_times ph{};
+    if (st == CFM_OK && p.phase_timers && p.variant != CFM_JACOBI_LAUNCH_PER_ITER)
+        st = cfm_jacobi_plan_phase_times(plan, &ph);
     cfm_jacobi_plan_destroy(plan);
     if (st != CFM_OK) return fail("jacobi run", st);
 
@@ -144,13 +151,24 @@ inline int run_jacobi(int argc, char **argv, int dims) {
                         p.nx, p.ny, p.nz, runtime_single, 1, secs, runtime_single / secs,
                         runtime_single / secs * 100);
     }
+    if (ph.workgroups > 0)
+        std::printf("phases (ms, mean over the kernel's %d workgroups): sweeps %.4f (largest %.4f) barrier %.4f "
+                    "(largest %.4f), total %.4f\n",
+                    ph.workgroups, ph.ms[CFM_PHASE_SPMV], ph.max_ms[CFM_PHASE_SPMV], ph.ms[CFM_PHASE_BARRIER],
+                    ph.max_ms[CFM_PHASE_BARRIER], ph.total_ms);
     if (json) {
         const double lups = (double)(p.nx - 2) * (p.ny - 2) * (dims == 2 ? 1 : p.nz - 2) * p.niter;
         std::printf("{\"version\": \"%s\", \"kernel_ms\": %.4f, \"glups\": %.3f, \"gbs\": %.1f, \"grid\": %d, "
-                    "\"block\": %d, \"passes\": %d, \"cooperative\": %d}\n",
+                    "\"block\": %d, \"passes\": %d, \"cooperative\": %d",
                     names[selection], t.kernel_ms, lups / (t.kernel_ms * 1e6),
                     lups * 2 * esz / (t.kernel_ms * 1e6), t.grid_blocks, t.block_threads, t.outer_iterations,
                     t.cooperative);
+        if (ph.workgroups > 0)
+            std::printf(", \"phases\": {\"workgroups\": %d, \"sweeps_ms\": %.4f, \"sweeps_max_ms\": %.4f, "
+                        "\"barrier_ms\": %.4f, \"barrier_max_ms\": %.4f, \"total_ms\": %.4f}",
+                        ph.workgroups, ph.ms[CFM_PHASE_SPMV], ph.max_ms[CFM_PHASE_SPMV], ph.ms[CFM_PHASE_BARRIER],
+                        ph.max_ms[CFM_PHASE_BARRIER], ph.total_ms);
+        std::printf("}\n");
     }
     return 0;
 }

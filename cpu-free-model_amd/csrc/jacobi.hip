@@ -630,6 +630,7 @@ struct T2Tile {
         T row[WY][TW];         // level-1 own row of each wave (plane p)
         T l1e[WY][WX][2];      // its first (lane 0) and last (lane 63) element
         T l0e[WY][WX][2][2];   // level-0 column rows of plane p+1: [slot][first, last]
+        T l0x[WY][WX][2][64];  // the same edges, one slot per lane (sweep3d_t2r: unmasked stores)
     };
 };
 
@@ -656,7 +657,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     static_assert(WY >= 2, "the top and bottom wave must differ");
     static_assert(!FULL || ALIGNED, "full-width rows are 16-byte aligned");
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: roles become scalar branches
     const int wx = wave % WX, wy = wave / WX;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const size_t plane = (size_t)nx * ny;
@@ -908,6 +909,301 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
     __syncthreads();  // the LDS buffers are reused by the next unit
 }
 
+// Two sweeps per pass with a two-step load lookahead.  In sweep3d_t2 the
+// level-0 window slides by register moves, and moving the register a load is
+// still writing forces the wave to wait for that load at the end of the very
+// step that issued it, so at most one step of compute covers the memory
+// latency whatever the lookahead distance.  Here the column rows of plane
+// p+3 land in a two-slot ring whose slot is fixed at compile time (the plane
+// loop is unrolled by two), and only the slot issued one step earlier is
+// moved into the window: those loads stay in flight across the next step.
+// The side rows (plane p+1, mostly L2 hits: the neighbouring wave streamed
+// them) are issued first in each step, so waiting for them at its end leaves
+// the ring loads issued after them in flight (the load counter is in order).
+// Rows come through a buffer resource: a lane's column is a fixed vector
+// offset, the row a scalar offset, and rows or planes outside the field get
+// an offset beyond the range check, which returns zeros, so every wave issues
+// the same unconditional loads each step (the compiler's wait counts stay
+// exact).  Same arithmetic, order and results as sweep3d_t2 (16-byte aligned
+// rows; fields below 2 GiB).
+template <int S>
+struct Slot {
+    static constexpr int value = S;
+};
+
+template <typename T, int WX, int WY, int VXT, bool FULL, bool PER>
+__device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
+                                            int ty, int zc, bool single, bool want0, bool want1, double &n0,
+                                            double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
+    using Tl = T2Tile<T, WX, WY, VXT>;
+    constexpr int VX = Tl::VX;
+    constexpr int E = 16 / sizeof(T);
+    static_assert(WY >= 2, "the top and bottom wave must differ");
+    static_assert(VX % E == 0, "whole 16-byte vectors");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: roles become scalar branches
+    const int wx = wave % WX, wy = wave / WX;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const size_t plane = (size_t)nx * ny;
+    const int x = wx * Tl::SEG + lane * VX;
+    const int y = ty * WY + wy;
+    const int zb = 1 + zc * A.chunk_len;
+    const int ze = min(zb + A.chunk_len, nz - 1);
+    if (zb >= ze) return;  // uniform over the workgroup
+    const T c = stencil_coeff3d<T>();
+    const bool top = wy == 0, bot = wy == WY - 1;
+    const bool lane_w = lane == 0, lane_e = lane == 63;
+    const int yc1 = top ? y - 1 : y + 1;
+    const bool has_h = top || bot;
+    const int ys0 = top ? y - 2 : y - 1;
+    const int ys1 = bot ? y + 2 : y + 1;
+    bool xin[VX];
+#pragma unroll
+    for (int k = 0; k < VX; ++k) xin[k] = x + k >= 1 && x + k <= nx - 2;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(plane * nz * sizeof(T)), 0x00020000);
+    constexpr unsigned kOut = 0x80000000u;  // beyond every field served here: the load returns zeros
+
+    T col[3][2][VX];  // level-0 column rows (own, tile halo) of planes p-1, p, p+1
+    T rc[2][2][VX];   // ring: column rows of the two planes in flight
+    T sid[2][VX];     // side rows of plane p
+    T rsd[2][VX];     // side rows of plane p+1 (in flight during a step)
+    T l1o[3][VX];
+    T l1h[2][VX];
+
+    // a lane's column as a vector offset, the row as a scalar one; outside the field the vector
+    // offset carries kOut and the scalar one is 0
+    const unsigned vo_in = (unsigned)x * (unsigned)sizeof(T), vo_out = vo_in | kOut;
+    auto ldrow = [&](int z, int yy, bool use, T (&v)[VX]) {
+        const bool ok = use && z >= 0 && z < nz && yy >= 0 && yy < ny;  // wave-uniform
+        const unsigned so = ok ? (unsigned)(((unsigned)z * (unsigned)ny + (unsigned)yy) * (unsigned)nx) *
+                                     (unsigned)sizeof(T)
+                               : 0u;
+        const unsigned vo = ok ? vo_in : vo_out;
+#pragma unroll
+        for (int q = 0; q < VX / E; ++q) {
+            const auto t = __builtin_bit_cast(typename Vec16<T>::type,
+                                              __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(vo + 16 * q), (int)so, 0));
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[q * E + e] = t[e];
+        }
+    };
+    auto ld_col = [&](int z, T (&v)[2][VX]) {
+        ldrow(z, y, true, v[0]);
+        ldrow(z, yc1, has_h, v[1]);
+    };
+    auto ld_side = [&](int z, T (&v)[2][VX]) {
+        ldrow(z, ys0, true, v[0]);
+        ldrow(z, ys1, true, v[1]);
+    };
+    // x edges of the level-0 rows of plane p+1: every lane stores one value into its own slot
+    // (lane 0 its first column, the others their last; no masked stores), the neighbouring wave
+    // reads the slot of lane 63 or lane 0 (a broadcast read)
+    auto put_l0e = [&](typename Tl::Buf &B, const T (&v)[2][VX]) {
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) B.l0x[wy][wx][sl][lane] = lane_w ? v[sl][0] : v[sl][VX - 1];
+    };
+    // west / east neighbours of the lane's first and last column: DPP inside the wave, the
+    // neighbouring wave's edge at lanes 0 / 63 (zero beyond the tile: only boundary columns see it)
+    // (both candidates computed first: a DPP move inside one arm of the select would make it a
+    // divergent branch)
+    auto xnb = [&](const T (&cr)[VX], T wn, T en, T &west, T &east) {
+        const T dw = lane_up1(cr[VX - 1]);
+        const T de = lane_dn1(cr[0]);
+        const T ww = wx > 0 ? wn : T(0);
+        const T ee = wx < WX - 1 ? en : T(0);
+        west = lane_w ? ww : dw;
+        east = lane_e ? ee : de;
+    };
+    auto l1row = [&](const typename Tl::Buf &B, int sl, int yy, bool pin, const T (&cr)[VX], const T (&N)[VX],
+                     const T (&S)[VX], const T (&U)[VX], const T (&D)[VX], T (&out)[VX], bool norm) {
+        T west, east;
+        xnb(cr, B.l0x[wy][wx > 0 ? wx - 1 : 0][sl][63], B.l0x[wy][wx < WX - 1 ? wx + 1 : 0][sl][0], west, east);
+        if (pin && yy >= 1 && yy <= ny - 2) {  // wave-uniform
+#pragma unroll
+            for (int k = 0; k < VX; ++k) {
+                const T Ev = (k < VX - 1) ? cr[k + 1] : east;
+                const T Wv = (k > 0) ? cr[k - 1] : west;
+                const T v = c * (Ev + Wv + N[k] + S[k] + U[k] + D[k]);
+                out[k] = (FULL && k > 0 && k < VX - 1) ? v : (xin[k] ? v : cr[k]);
+                if (norm) {  // wave-uniform; adding +0.0 outside the interior leaves the sum's bits alone
+                    const double d = (double)v - (double)cr[k];
+                    n0 += xin[k] ? d * d : 0.0;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) out[k] = cr[k];
+        }
+    };
+    auto level1 = [&](const typename Tl::Buf &Bp, bool pin, bool nrm) {
+        if (top) {
+            l1row(Bp, 0, y, pin, col[1][0], sid[1], col[1][1], col[2][0], col[0][0], l1o[2], nrm);
+            l1row(Bp, 1, y - 1, pin, col[1][1], col[1][0], sid[0], col[2][1], col[0][1], l1h[1], false);
+        } else if (bot) {
+            l1row(Bp, 0, y, pin, col[1][0], col[1][1], sid[0], col[2][0], col[0][0], l1o[2], nrm);
+            l1row(Bp, 1, y + 1, pin, col[1][1], sid[1], col[1][0], col[2][1], col[0][1], l1h[1], false);
+        } else {
+            l1row(Bp, 0, y, pin, col[1][0], sid[1], sid[0], col[2][0], col[0][0], l1o[2], nrm);
+        }
+    };
+    auto store_row = [&](int z, const T (&out)[VX]) {
+        T *drow = dst + (size_t)z * plane + (size_t)y * nx;
+        if (FULL || (x >= 1 && x + VX - 1 <= nx - 2)) {
+#pragma unroll
+            for (int q = 0; q < VX / E; ++q) {
+                typename Vec16<T>::type t;
+#pragma unroll
+                for (int e = 0; e < E; ++e) t[e] = out[q * E + e];
+                store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E), t);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k)
+                if (xin[k]) drow[x + k] = out[k];
+        }
+    };
+    auto store_copy = [&](int z, const T (&out)[VX]) {
+        if (y >= ny) return;  // wave-uniform
+        T *drow = dst + (size_t)z * plane + (size_t)y * nx;
+        if (FULL) {
+#pragma unroll
+            for (int q = 0; q < VX / E; ++q) {
+                typename Vec16<T>::type t;
+#pragma unroll
+                for (int e = 0; e < E; ++e) t[e] = out[q * E + e];
+                store16(reinterpret_cast<typename Vec16<T>::type *>(drow + x + q * E), t);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k)
+                if (x + k < nx) drow[x + k] = out[k];
+        }
+    };
+
+    int p0 = single ? zb : zb - 1;
+    const int p1 = single ? ze - 1 : ze;  // inclusive
+    if (PER && !single && zb == 1) {
+        // level 1 on plane 0 = level 1 on plane nz-2, from stored planes nz-3 .. nz-1
+        ld_col(nz - 3, col[0]);
+        ld_col(nz - 2, col[1]);
+        ld_col(nz - 1, col[2]);
+        ld_side(nz - 2, sid);
+        put_l0e(lds[0], col[1]);
+        __syncthreads();
+        level1(lds[0], true, false);
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+            l1o[1][k] = l1o[2][k];
+            l1h[0][k] = l1h[1][k];
+        }
+        p0 = 1;
+    }
+    ld_col(p0 - 1, col[0]);
+    ld_col(p0, col[1]);
+    ld_col(p0 + 1, col[2]);
+    ld_side(p0, sid);
+    ld_col(p0 + 2, rc[1]);  // slot of step p0 + 1, moved into the window at the end of step p0
+    put_l0e(lds[p0 % 3], col[1]);
+    __syncthreads();
+
+    auto step = [&](int p, auto slot) {
+        constexpr int S = decltype(slot)::value;
+        ld_side(p + 1, rsd);    // into the window at the end of this step (issued first)
+        ld_col(p + 3, rc[S]);   // into the window at the end of step p + 1
+        typename Tl::Buf &Bp = lds[p % 3];
+        bool pin = p >= 1 && p <= nz - 2;
+        if (PER && !single && p == nz - 1) {
+            // level 1 on plane nz-1 = level 1 on plane 1, from stored planes 0 .. 2
+            ld_col(0, col[0]);
+            ld_col(1, col[1]);
+            ld_col(2, col[2]);
+            ld_side(1, sid);
+            put_l0e(Bp, col[1]);
+            __syncthreads();
+            pin = true;
+        }
+        const bool nrm = want0 && p >= zb && p < ze;
+        level1(Bp, pin, nrm);
+        typename Tl::Buf &Bn = lds[(p + 1) % 3];
+        if (single) {
+            if (y >= 1 && y <= ny - 2) store_row(p, l1o[2]);
+            if (PER) {
+                if (p == nz - 2) store_copy(0, l1o[2]);
+                if (p == 1) store_copy(nz - 1, l1o[2]);
+            }
+            put_l0e(Bn, col[2]);
+            __syncthreads();
+        } else {
+            st_vec<T, VX>(&Bp.row[wy][x], l1o[2]);  // (its x edges are read from here too)
+            put_l0e(Bn, col[2]);
+            __syncthreads();
+            const int zo = p - 1;
+            if (zo >= zb && zo >= 1 && zo <= nz - 2 && y >= 1 && y <= ny - 2) {  // wave-uniform
+                const typename Tl::Buf &R = lds[(p + 2) % 3];
+                T N[VX], Sv[VX];
+                if (top) {
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) Sv[k] = l1h[0][k];
+                } else {
+                    ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x], Sv);
+                }
+                if (bot) {
+#pragma unroll
+                    for (int k = 0; k < VX; ++k) N[k] = l1h[0][k];
+                } else {
+                    ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x], N);
+                }
+                const T (&cr)[VX] = l1o[1];
+                T west, east;
+                xnb(cr, R.row[wy][wx > 0 ? wx * Tl::SEG - 1 : 0], R.row[wy][wx < WX - 1 ? (wx + 1) * Tl::SEG : 0],
+                    west, east);
+                T out[VX];
+#pragma unroll
+                for (int k = 0; k < VX; ++k) {
+                    const T Ev = (k < VX - 1) ? cr[k + 1] : east;
+                    const T Wv = (k > 0) ? cr[k - 1] : west;
+                    const T v = c * (Ev + Wv + N[k] + Sv[k] + l1o[2][k] + l1o[0][k]);
+                    out[k] = (FULL && k > 0 && k < VX - 1) ? v : (xin[k] ? v : cr[k]);
+                    if (want1) {
+                        const double d = (double)v - (double)cr[k];
+                        n1 += xin[k] ? d * d : 0.0;
+                    }
+                }
+                store_row(zo, out);
+                if (PER) {
+                    if (zo == nz - 2) store_copy(0, out);
+                    if (zo == 1) store_copy(nz - 1, out);
+                }
+            } else if (PER && zo >= zb && (zo == 1 || zo == nz - 2) && (y == 0 || y == ny - 1)) {
+                if (zo == nz - 2) store_copy(0, l1o[1]);
+                if (zo == 1) store_copy(nz - 1, l1o[1]);
+            }
+        }
+        // ---- slide: the window takes the ring slot issued one step earlier
+#pragma unroll
+        for (int k = 0; k < VX; ++k) {
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                col[0][sl][k] = col[1][sl][k];
+                col[1][sl][k] = col[2][sl][k];
+                col[2][sl][k] = rc[S ^ 1][sl][k];
+                sid[sl][k] = rsd[sl][k];
+            }
+            l1o[0][k] = l1o[1][k];
+            l1o[1][k] = l1o[2][k];
+            l1h[0][k] = l1h[1][k];
+        }
+    };
+    for (int p = p0;;) {
+        step(p, Slot<0>{});
+        if (++p > p1) break;
+        step(p, Slot<1>{});
+        if (++p > p1) break;
+    }
+    __syncthreads();  // the LDS buffers are reused by the next unit
+}
+
 template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
     using Tl = T2Tile<T, WX, WY, VXT>;
@@ -932,8 +1228,13 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
         for (int u = lb; u < nunits; u += g) {
             const int zc = u / A.nty;
             const int ty = u - zc * A.nty;
-            sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1,
-                                                           bufs);
+            // 16-byte aligned rows: the two-step lookahead through buffer loads (fields below 2 GiB,
+            // which the host guarantees for the ALIGNED builds)
+            if constexpr (ALIGNED)
+                sweep3d_t2r<T, WX, WY, VXT, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1, bufs);
+            else
+                sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0,
+                                                               n1, bufs);
         }
         if (want0) {
             const double s = block_sum_waves<Tl::BLOCK / 64>(n0, red);
@@ -1794,12 +2095,14 @@ static KernelFn pick_t2_kernel(bool aligned, bool full, int block, bool periodic
         // aligned build handles nx = 512 as well
         return pick_t2_shape<T, 4, 4, 2, TM, false>(aligned, full);
     } else {
+        // (the full-width periodic build of this shape spills to scratch: the general aligned one
+        // serves nx = 512 too)
         if (block == 512)
-            return periodic ? pick_t2_shape<T, 1, 8, 8, TM, true, true>(aligned, full)
+            return periodic ? pick_t2_shape<T, 1, 8, 8, TM, false, true>(aligned, full)
                             : pick_t2_shape<T, 1, 8, 8, TM>(aligned, full);
         if (periodic) return nullptr;
-        // (the timed full-width build of this shape spills: its timed run uses the general build)
-        return pick_t2_shape<T, 2, 8, 4, TM, !TM>(aligned, full);
+        // (the lookahead builds of this shape spill to scratch: it keeps the element-load build)
+        return (KernelFn)jacobi3d_t2_kernel<T, 2, 8, 4, false, false, false, TM>;
     }
 }
 
@@ -1911,13 +2214,16 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
         // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428
         // GLUP/s (2 columns, 1024 threads); FP32 one wavefront across x (8 columns per lane)
         // 882 vs 844 (4 columns, 1024 threads)
+        // 16-byte aligned rows use buffer loads with 32-bit offsets (the two-step load lookahead,
+        // 512^3 FP64 477 -> 545 GLUP/s): fields of 2 GiB and more take the unaligned build
         c->block = p->block_threads > 0 ? p->block_threads : 512;
-        const bool full = p->nx == kT2Width;
+        const bool al = aligned && c->elems * c->elem_size < 0x80000000ull;
+        const bool full = p->nx == kT2Width && al;
         const bool per = p->bc == CFM_BC_PERIODIC;
-        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(aligned, full, c->block, per)
-                              : pick_t2_kernel<double, false>(aligned, full, c->block, per))
-                        : (tm ? pick_t2_kernel<float, true>(aligned, full, c->block, per)
-                              : pick_t2_kernel<float, false>(aligned, full, c->block, per));
+        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(al, full, c->block, per)
+                              : pick_t2_kernel<double, false>(al, full, c->block, per))
+                        : (tm ? pick_t2_kernel<float, true>(al, full, c->block, per)
+                              : pick_t2_kernel<float, false>(al, full, c->block, per));
     } else {
         c->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
         c->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, c->rows, f64);

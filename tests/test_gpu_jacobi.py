@@ -246,6 +246,17 @@ def test_two_sweeps_per_pass_periodic_units(gpu, orc, norm_every, chunks, grid):
     _check(gpu, orc, init, 9, 0, norm_every=norm_every, time_block=2, chunks=chunks, grid_blocks=grid)
 
 
+@pytest.mark.parametrize("bc", [1, 0])
+def test_two_sweeps_per_pass_field_of_2gib_and_more(gpu, orc, bc):
+    """Fields of 2 GiB and more take the element-load build (the lookahead
+    build addresses rows with 32-bit buffer offsets): 512 x 512 x 1026 FP64,
+    three sweeps (one two-sweep pass, one single), bit for bit."""
+    init = _rand((1026, 512, 512), np.float64, 31)
+    assert init.nbytes >= 2 ** 31
+    t = _check(gpu, orc, init, 3, bc, norm_every=2)
+    assert t["outer_iterations"] == 2
+
+
 def test_reference_mode_3d_full_size_bitexact(gpu, orc):
     """The jacobi3D CLI's default mode at the benchmark size: 512^3 FP64, sin
     x faces, periodic z halos, 1000 sweeps (two per HBM pass), update norm

@@ -83,10 +83,21 @@ __device__ __forceinline__ double sub_sum(double v, double *lds) {
 // Reduce nchunks partials (fixed order) -> same value in every thread: each
 // sub-block's thread t adds partials t, t+256, ... and the sub-block tree
 // combines them (every sub-block computes the identical value).
+// The loads go out eight at a time before the adds (same order of additions, so
+// the same bits): after a device barrier they come from the memory-side cache,
+// and one latency per eight partials replaces one per partial.
 template <int SUB>
 __device__ __forceinline__ double reduce_partials(const double *part, int nchunks, double *lds) {
     double a = 0.0;
-    for (int j = threadIdx.x & (kThreads - 1); j < nchunks; j += kThreads) a = a + part[j];
+    int j = threadIdx.x & (kThreads - 1);
+    for (; j + 7 * kThreads < nchunks; j += 8 * kThreads) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[j + u * kThreads];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a = a + v[u];
+    }
+    for (; j < nchunks; j += kThreads) a = a + part[j];
     return sub_sum<SUB>(a, lds);
 }
 

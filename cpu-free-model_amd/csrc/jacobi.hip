@@ -24,6 +24,8 @@
 //     against the CPU oracle.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -131,6 +133,42 @@ __device__ __forceinline__ T lane_dn1(T v) {
         return r.t;
     } else {
         return __shfl_down(v, 1, 64);
+    }
+}
+
+// The same shifts with zeros shifted in at lane 0 / lane 63 (bound_ctrl): no copy
+// of the old value into the destination first, for callers that replace those
+// two lanes anyway.
+template <typename T>
+__device__ __forceinline__ T lane_up1z(T v) {
+    static_assert(sizeof(T) == 8 || sizeof(T) == 4, "4- or 8-byte values");
+    if constexpr (sizeof(T) == 8) {
+        union {
+            T t;
+            int i[2];
+        } a, r;
+        a.t = v;
+        r.i[0] = __builtin_amdgcn_mov_dpp(a.i[0], 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+        r.i[1] = __builtin_amdgcn_mov_dpp(a.i[1], 0x138, 0xf, 0xf, true);
+        return r.t;
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+    }
+}
+template <typename T>
+__device__ __forceinline__ T lane_dn1z(T v) {
+    static_assert(sizeof(T) == 8 || sizeof(T) == 4, "4- or 8-byte values");
+    if constexpr (sizeof(T) == 8) {
+        union {
+            T t;
+            int i[2];
+        } a, r;
+        a.t = v;
+        r.i[0] = __builtin_amdgcn_mov_dpp(a.i[0], 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
+        r.i[1] = __builtin_amdgcn_mov_dpp(a.i[1], 0x130, 0xf, 0xf, true);
+        return r.t;
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
     }
 }
 
@@ -1004,16 +1042,15 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         for (int sl = 0; sl < 2; ++sl) B.l0x[wy][wx][sl][lane] = lane_w ? v[sl][0] : v[sl][VX - 1];
     };
     // west / east neighbours of the lane's first and last column: DPP inside the wave, the
-    // neighbouring wave's edge at lanes 0 / 63 (zero beyond the tile: only boundary columns see it)
-    // (both candidates computed first: a DPP move inside one arm of the select would make it a
-    // divergent branch)
+    // neighbouring wave's edge at lanes 0 / 63.  At the tile's own x ends (one tile spans the
+    // rows) those lanes hold column 0 or columns >= nx-1, whose outputs keep their stored value,
+    // so whatever the clamped read there returns is never used.  Both candidates are computed
+    // first (a DPP move inside one arm of the select would make it a divergent branch).
     auto xnb = [&](const T (&cr)[VX], T wn, T en, T &west, T &east) {
-        const T dw = lane_up1(cr[VX - 1]);
-        const T de = lane_dn1(cr[0]);
-        const T ww = wx > 0 ? wn : T(0);
-        const T ee = wx < WX - 1 ? en : T(0);
-        west = lane_w ? ww : dw;
-        east = lane_e ? ee : de;
+        const T dw = lane_up1z(cr[VX - 1]);
+        const T de = lane_dn1z(cr[0]);
+        west = lane_w ? wn : dw;
+        east = lane_e ? en : de;
     };
     auto l1row = [&](const typename Tl::Buf &B, int sl, int yy, bool pin, const T (&cr)[VX], const T (&N)[VX],
                      const T (&S)[VX], const T (&U)[VX], const T (&D)[VX], T (&out)[VX], bool norm) {
@@ -1107,14 +1144,15 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     put_l0e(lds[p0 % 3], col[1]);
     __syncthreads();
 
-    auto step = [&](int p, auto slot) {
+    // one plane step; WRAP: the periodic last step of the chunk ending at plane nz-2, where level
+    // 1 on plane nz-1 is level 1 on plane 1, from stored planes 0 .. 2 (run after the loop, so
+    // the loop body carries none of it)
+    auto step = [&](int p, auto slot, auto wrap) {
         constexpr int S = decltype(slot)::value;
-        ld_side(p + 1, rsd);    // into the window at the end of this step (issued first)
-        ld_col(p + 3, rc[S]);   // into the window at the end of step p + 1
+        constexpr bool WRAP = decltype(wrap)::value;
         typename Tl::Buf &Bp = lds[p % 3];
         bool pin = p >= 1 && p <= nz - 2;
-        if (PER && !single && p == nz - 1) {
-            // level 1 on plane nz-1 = level 1 on plane 1, from stored planes 0 .. 2
+        if constexpr (WRAP) {
             ld_col(0, col[0]);
             ld_col(1, col[1]);
             ld_col(2, col[2]);
@@ -1122,6 +1160,9 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
             put_l0e(Bp, col[1]);
             __syncthreads();
             pin = true;
+        } else {
+            ld_side(p + 1, rsd);   // into the window at the end of this step (issued first)
+            ld_col(p + 3, rc[S]);  // into the window at the end of step p + 1
         }
         const bool nrm = want0 && p >= zb && p < ze;
         level1(Bp, pin, nrm);
@@ -1180,6 +1221,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
                 if (zo == 1) store_copy(nz - 1, l1o[1]);
             }
         }
+        if constexpr (WRAP) return;  // the chunk's last step
         // ---- slide: the window takes the ring slot issued one step earlier
 #pragma unroll
         for (int k = 0; k < VX; ++k) {
@@ -1195,14 +1237,20 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
             l1h[0][k] = l1h[1][k];
         }
     };
-    for (int p = p0;;) {
-        step(p, Slot<0>{});
-        if (++p > p1) break;
-        step(p, Slot<1>{});
-        if (++p > p1) break;
+    const bool wrap_top = PER && !single && ze == nz - 1;
+    const int pl = wrap_top ? p1 - 1 : p1;
+    for (int p = p0; p <= pl;) {
+        step(p, Slot<0>{}, std::false_type{});
+        if (++p > pl) break;
+        step(p, Slot<1>{}, std::false_type{});
+        if (++p > pl) break;
+    }
+    if constexpr (PER) {
+        if (wrap_top) step(p1, Slot<0>{}, std::true_type{});
     }
     __syncthreads();  // the LDS buffers are reused by the next unit
 }
+
 
 template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {

@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--profile-batch", action="store_true",
                     help="profiling run: the copy probe, then one batched solve of the 256-system "
                          "variable-coefficient workload (default shape, one dispatch); prints its figures and exits")
+    ap.add_argument("--profile-unet", action="store_true",
+                    help="profiling run of the warm-start network at 32 x 64^3: forward passes and training "
+                         "steps (for rocprofv3 --kernel-trace); prints their wall times and exits")
     ap.add_argument("--profile-cg", action="store_true",
                     help="profiling run: the copy probe, then one 128^3 Poisson solve per CG method and "
                          "schedule (one dispatch each); prints the iteration counts and exits")
@@ -275,6 +278,28 @@ def cg_batch_profile_run(cfm, nsys, lmin=32, lmax=96, seed=2024):
             "grid": t["grid_blocks"], "block": t["block_threads"]}
 
 
+def unet_profile_run(cfm, n=64, batch=32, reps=5):
+    """The warm-start network's forward pass and Adam training step on a
+    batch of 32 random 64^3 inputs (He-uniform weights): wall time per call,
+    host transfers of the input, output and target included; the kernel
+    times come from a kernel trace of this run."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((batch, 2, n ** 3)).astype(np.float32)
+    y = rng.standard_normal((batch, n ** 3)).astype(np.float32)
+    out = {"n": n, "batch": batch}
+    with cfm.UNet(n, batch, params=cfm.unet_init_params(1)) as net:
+        for name, call in (("forward", lambda: net.forward(x)), ("train_step", lambda: net.train_step(x, y, 1e-4))):
+            call()
+            cfm.lib.cfm_device_synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            cfm.lib.cfm_device_synchronize()
+            out[f"{name}_wall_ms"] = round(1000 * (time.perf_counter() - t0) / reps, 2)
+    return out
+
+
 def warmstart_figures(nsys):
     """Learned warm start on held-out 64^3 variable-coefficient systems: one
     batched plan solved from x0 = 0 and from the trained network's guess
@@ -299,6 +324,9 @@ def main():
         gbs, ms = cfm.bandwidth_probe(1 << 30, 5)
         print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1)}), flush=True)
         print(json.dumps(cg_batch_profile_run(cfm, args.batch_systems)), flush=True)
+        return
+    if args.profile_unet:
+        print(json.dumps(unet_profile_run(cfm)), flush=True)
         return
     if args.profile_cg:
         gbs, ms = cfm.bandwidth_probe(1 << 30, 5)

@@ -31,6 +31,7 @@ namespace unet {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(bf16 v) { return (float)v; }
@@ -47,20 +48,23 @@ __device__ __forceinline__ int tap_off(int tap, int &dz, int &dy, int &dx) {
 // Y = conv3x3x3(X) + bias, optional ReLU.  W is [Cout][Kp] BF16 (k = tap*Cin+ci,
 // zero padded to Kp, a multiple of 32).
 // LDS-staged forward conv for the layer shapes of the network.  A workgroup
-// owns 16 (x) x 4 (y) x TZ (z) output points of one batch element; the input
+// owns 16 (x) x TY (y) x TZ (z) output points of one batch element; the input
 // block with its one-point halo and all CIN channels is staged in LDS once
 // (zeros outside the grid), so a lane's B fragment (8 channels of one tap at
-// one point) is a single ds_read_b128 with no bounds checks.  Wavefront w
-// computes TZ == 2: rows y0 + 2(w&1) + {0,1} of plane z0 + (w>>1) (NT = 2);
-// TZ == 1: row y0 + w (NT = 1), times MT channel tiles.
-template <int MT, int CIN, int TZ>
+// one point) is a single ds_read_b128 with no bounds checks.  Each of the
+// four wavefronts computes NT = TY*TZ/4 rows of 16 points (rows w*NT .. of the
+// tile, y fastest) for all MT channel tiles: every weight fragment (read from
+// the L1/L2-resident weights) feeds NT MFMAs and every input fragment MT.
+// The four accumulators of a lane (output channels 4g..4g+3 of one point) are
+// biased, rectified, rounded to BF16 and written as one 8-byte store.
+template <int MT, int CIN, int TY, int TZ>
 __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
                                                             int Kp, const float *__restrict__ bias,
                                                             bf16 *__restrict__ Y, int n, int nbx, int nby, int nbz,
                                                             int relu) {
-    constexpr int HX = 18, HY = 6, HZ = TZ + 2, NV = CIN / 8;
-    constexpr int NT = TZ == 2 ? 2 : 1;
-    static_assert(CIN % 8 == 0 && (TZ == 1 || TZ == 2), "shape");
+    constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2, NV = CIN / 8;
+    constexpr int NT = TY * TZ / 4;
+    static_assert(CIN % 8 == 0 && (TY * TZ) % 4 == 0, "shape");
     __shared__ bf16x8 blk[HZ * HY * HX * NV];
     int bid = blockIdx.x;
     const int bx = bid % nbx;
@@ -69,7 +73,7 @@ __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restri
     bid /= nby;
     const int bz = bid % nbz;
     const int b = bid / nbz;
-    const int x0 = bx * 16, y0 = by * 4, z0 = bz * TZ;
+    const int x0 = bx * 16, y0 = by * TY, z0 = bz * TZ;
     const bf16x8 zero8 = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f,
                           (bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
     for (int v = threadIdx.x; v < HZ * HY * HX * NV; v += 256) {
@@ -85,12 +89,14 @@ __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restri
         blk[v] = val;
     }
     __syncthreads();
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int nl = lane & 15, g = lane >> 4;
-    const int tz = TZ == 2 ? (w >> 1) : 0;
-    int ty[NT];
+    int ty[NT], tz[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) ty[t] = TZ == 2 ? 2 * (w & 1) + t : w;
+    for (int t = 0; t < NT; ++t) {
+        ty[t] = (w * NT + t) % TY;
+        tz[t] = (w * NT + t) / TY;
+    }
     constexpr int K = 27 * CIN;
     f32x4 acc[MT][NT];
 #pragma unroll
@@ -108,29 +114,30 @@ __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restri
         const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // offsets + 1
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            bf16x8 bv = blk[(((tz + dz) * HY + (ty[t] + dy)) * HX + (nl + dx)) * NV + ci8];
+            bf16x8 bv = blk[(((tz[t] + dz) * HY + (ty[t] + dy)) * HX + (nl + dx)) * NV + ci8];
             if (!kin) bv = zero8;
 #pragma unroll
             for (int m = 0; m < MT; ++m) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv, acc[m][t], 0, 0, 0);
         }
     }
-    // D[4g+i][nl]: output channel m*16 + 4g + i of point (x0 + nl, y0 + ty[t], z0 + tz)
+    // D[4g+i][nl]: output channel m*16 + 4g + i of point (x0 + nl, y0 + ty[t], z0 + tz[t])
     constexpr int Cout = MT * 16;
-    const int x = x0 + nl, z = z0 + tz;
+    const int x = x0 + nl;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int y = y0 + ty[t];
+        const int y = y0 + ty[t], z = z0 + tz[t];
         if (x >= n || y >= n || z >= n) continue;
         const size_t p = (((size_t)b * n + z) * n + y) * n + x;
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
+            bf16x4 o;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int co = m * 16 + 4 * g + i;
-                float v = acc[m][t][i] + (bias ? bias[co] : 0.f);
+                float v = acc[m][t][i] + (bias ? bias[m * 16 + 4 * g + i] : 0.f);
                 if (relu && v < 0.f) v = 0.f;
-                Y[p * Cout + co] = f2bf(v);
+                o[i] = f2bf(v);
             }
+            *reinterpret_cast<bf16x4 *>(Y + p * Cout + m * 16 + 4 * g) = o;
         }
     }
 }
@@ -750,10 +757,14 @@ extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
 template <int MT, int CIN>
 static void launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
                             const float *bias, bf16 *Y, int relu) {
-    constexpr int TZ = CIN * 2 * 18 * 6 * 4 <= 65536 ? 2 : 1;  // the staged block fits 64 KiB of LDS
-    const int nbx = (n + 15) / 16, nby = (n + 3) / 4, nbz = (n + TZ - 1) / TZ;
+    // tile 16 x TY x TZ points; the staged block stays within 64 KiB of LDS: up to 32 input
+    // channels 16 x 8 x 2 (four rows per wavefront), 48-64 channels 16 x 4 x 2, 96 channels 16 x 4 x 1
+    constexpr int TY = CIN <= 32 ? 8 : 4;
+    constexpr int TZ = CIN <= 64 ? 2 : 1;
+    static_assert((TZ + 2) * (TY + 2) * 18 * CIN * 2 <= 65536, "staged block above 64 KiB");
+    const int nbx = (n + 15) / 16, nby = (n + TY - 1) / TY, nbz = (n + TZ - 1) / TZ;
     const dim3 grid((unsigned)(nbx * nby * nbz * batch));
-    hipLaunchKernelGGL((conv3_fwd_lds_kernel<MT, CIN, TZ>), grid, dim3(256), 0, st, X, W, kp, bias, Y, n, nbx, nby,
+    hipLaunchKernelGGL((conv3_fwd_lds_kernel<MT, CIN, TY, TZ>), grid, dim3(256), 0, st, X, W, kp, bias, Y, n, nbx, nby,
                        nbz, relu);
 }
 

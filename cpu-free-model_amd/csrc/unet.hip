@@ -32,6 +32,7 @@ namespace unet {
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(bf16 v) { return (float)v; }
@@ -292,6 +293,159 @@ __global__ void __launch_bounds__(256) conv3_wgrad_lds_kernel(const bf16 *__rest
         }
 }
 
+// Weight gradient for every tap at once, for grids whose x rows tile 64-point
+// blocks (n in 16, 32, 64): workgroup s computes partial[s][co][tap*CIN + ci]
+// over its slab.  Per block of 64 points (R = 64/n whole x rows of one plane)
+// it stages dZ once, transposed [co][point], and the 3 x (R+2) input rows
+// around the block once, transposed [ci][x] with a zero halo column at each
+// end (x = -1 and x = n): the tap-shifted input of every tap is then a window
+// of those rows, so nothing is staged per tap.  Wavefront w owns the taps of
+// WROWS (dz, dy) rows; a lane's B fragment (8 consecutive points of one input
+// channel, shifted by dx) comes from two aligned 16-byte LDS reads: dx = -1 is
+// the first, dx = +1 the dwords 1..4, dx = 0 a 2-byte byte-align of the pair.
+// Input channels below 16 are padded with zero rows (the first layer's 8).
+// KS > 1 splits the input channels over KS workgroups per slab (blockIdx.y):
+// each stages and computes CIN / KS of them, which keeps the accumulators of
+// the wide layers in registers.
+template <int CIN, int COUT, int WROWS, int KS = 1>
+__global__ void __launch_bounds__(64 * 9 / WROWS) conv3_wgrad_rows_kernel(const bf16 *__restrict__ Xg,
+                                                                        const bf16 *__restrict__ dZ, int Kp,
+                                                                        float *__restrict__ partial, int n, int npts,
+                                                                        int slab) {
+    constexpr int CINX = CIN;            // channels per point of the input tensor
+    constexpr int CW = CIN / KS;         // input channels of this workgroup
+    constexpr int CINP = CW < 16 ? 16 : CW;
+    constexpr int MT = COUT / 16, KT = CINP / 16, NWAVE = 9 / WROWS, NTH = 64 * NWAVE;
+    static_assert(CIN % KS == 0 && (KS == 1 || CW % 16 == 0), "channel split");
+    const int ci0 = blockIdx.y * CW;
+    const bf16 *X = Xg + ci0;
+    constexpr int DS = 72;  // dZ row stride (64 points + 8: spreads the 16 rows of a fragment over the banks)
+    static_assert(CINP % 16 == 0 && COUT % 16 == 0 && CW % 8 == 0 && 9 % WROWS == 0, "shape");
+    extern __shared__ __attribute__((aligned(16))) bf16 wg_lds[];
+    const int R = 64 / n, W = n + 8, ROWS = 3 * (R + 2);
+    bf16 *dzT = wg_lds;                 // [COUT][DS]
+    bf16 *xT = wg_lds + COUT * DS;      // [ROWS][CINP][W], x + 1 stored at column x + 1
+    const int s = blockIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nl = lane & 15, g = lane >> 4;
+    const bf16x8 zero8 = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f,
+                          (bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+    if (CW < CINP) {  // padded input channels: zero once, never overwritten
+        for (int v = threadIdx.x; v < ROWS * (CINP - CW) * W; v += NTH) {
+            const int c = CW + (v / W) % (CINP - CW), r = v / (W * (CINP - CW));
+            xT[((size_t)r * CINP + c) * W + v % W] = (bf16)0.0f;
+        }
+    }
+    f32x4 acc[WROWS][3][MT][KT];
+#pragma unroll
+    for (int a = 0; a < WROWS; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int k = 0; k < KT; ++k) acc[a][d][m][k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int n2 = n * n, n3 = n2 * n;
+    const int pbeg = s * slab, pend = min(npts, pbeg + slab);
+    for (int pb = pbeg; pb < pend; pb += 64) {
+        const int b = pb / n3, rem = pb - b * n3;
+        const int z = rem / n2, y0 = (rem / n) % n;
+        // stage dZ of the 64 points, transposed: two points per thread, so each LDS store
+        // writes one channel's pair of neighbouring points (4 bytes)
+        for (int v = threadIdx.x; v < 32 * (COUT / 8); v += NTH) {
+            const int j = 2 * (v / (COUT / 8)), c8 = v % (COUT / 8);
+            const u16x8 a0 = __builtin_bit_cast(u16x8, *reinterpret_cast<const bf16x8 *>(dZ + (size_t)(pb + j) * COUT + 8 * c8));
+            const u16x8 a1 =
+                __builtin_bit_cast(u16x8, *reinterpret_cast<const bf16x8 *>(dZ + (size_t)(pb + j + 1) * COUT + 8 * c8));
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                *reinterpret_cast<unsigned *>(&dzT[(8 * c8 + i) * DS + j]) = (unsigned)a0[i] | ((unsigned)a1[i] << 16);
+        }
+        // stage the 3 x (R+2) input rows around the block, transposed, x = -1 .. n (n + 2 columns,
+        // two per thread as above)
+        for (int v = threadIdx.x; v < ROWS * (n / 2 + 1) * (CW / 8); v += NTH) {
+            const int c8 = v % (CW / 8);
+            const int xj = 2 * ((v / (CW / 8)) % (n / 2 + 1));
+            const int r = v / ((CW / 8) * (n / 2 + 1));
+            const int zz = z + r / (R + 2) - 1, yy = y0 + r % (R + 2) - 1;
+            u16x8 a[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int xx = xj + e - 1;
+                bf16x8 val = zero8;
+                if (xx >= 0 && xx < n && yy >= 0 && yy < n && zz >= 0 && zz < n)
+                    val = *reinterpret_cast<const bf16x8 *>(X + ((((size_t)b * n + zz) * n + yy) * n + xx) * CINX + 8 * c8);
+                a[e] = __builtin_bit_cast(u16x8, val);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                *reinterpret_cast<unsigned *>(&xT[((size_t)r * CINP + 8 * c8 + i) * W + xj]) =
+                    (unsigned)a[0][i] | ((unsigned)a[1][i] << 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {  // two K steps of 32 points
+            const int q = 32 * h + 8 * g;  // this lane's first point in the block
+            const int rq = q / n, xg = q % n;
+            bf16x8 av[MT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const bf16x8 *>(&dzT[(m * 16 + nl) * DS + q]);
+#pragma unroll
+            for (int a = 0; a < WROWS; ++a) {
+                const int rw = w * WROWS + a;  // (dz, dy) row index 0..8
+                const int dz = rw / 3, dy = rw % 3;  // offsets + 1
+                const int r = dz * (R + 2) + rq + dy;
+#pragma unroll
+                for (int k = 0; k < KT; ++k) {
+                    const bf16 *src = &xT[((size_t)r * CINP + k * 16 + nl) * W + xg];
+                    const bf16x8 lo = *reinterpret_cast<const bf16x8 *>(src);
+                    const bf16x8 hi = *reinterpret_cast<const bf16x8 *>(src + 8);
+                    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+                    const u32x4 l4 = __builtin_bit_cast(u32x4, lo), h4 = __builtin_bit_cast(u32x4, hi);
+                    const u32x4 mid = {__builtin_amdgcn_alignbyte(l4[1], l4[0], 2),
+                                       __builtin_amdgcn_alignbyte(l4[2], l4[1], 2),
+                                       __builtin_amdgcn_alignbyte(l4[3], l4[2], 2),
+                                       __builtin_amdgcn_alignbyte(h4[0], l4[3], 2)};
+                    const u32x4 up = {l4[1], l4[2], l4[3], h4[0]};
+                    const bf16x8 bv[3] = {lo, __builtin_bit_cast(bf16x8, mid), __builtin_bit_cast(bf16x8, up)};
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int m = 0; m < MT; ++m)
+                            acc[a][d][m][k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[d], acc[a][d][m][k], 0,
+                                                                                      0, 0);
+                }
+            }
+        }
+        __syncthreads();  // the staged rows are rewritten next
+    }
+    // D[4g+i][nl] = dW[m*16 + 4g + i][tap*CIN + k*16 + nl], tap = 9 dz + 3 dy + dx (offsets + 1)
+    float *out = partial + (size_t)s * COUT * Kp;
+#pragma unroll
+    for (int a = 0; a < WROWS; ++a) {
+        const int rw = w * WROWS + a;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int tap = 3 * rw + d;
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int k = 0; k < KT; ++k) {
+                    const int ci = k * 16 + nl;
+                    if (ci < CW)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            out[(size_t)(m * 16 + 4 * g + i) * Kp + tap * CIN + ci0 + ci] = acc[a][d][m][k][i];
+                }
+        }
+    }
+    if (blockIdx.y == 0)
+    for (int e = threadIdx.x; e < COUT * (Kp - 27 * CIN); e += NTH) {  // the zero-padded K tail
+        const int co = e / (Kp - 27 * CIN), k = 27 * CIN + e % (Kp - 27 * CIN);
+        out[(size_t)co * Kp + k] = 0.f;
+    }
+}
+
 // out[i] = sum_s partial[s][i] (fixed order s = 0, 1, ...), i < count.  Eight
 // loads are issued before their eight in-order adds, so the memory latency is
 // paid once per eight slabs instead of once per slab.
@@ -310,6 +464,29 @@ __global__ void reduce_slabs_kernel(const float *__restrict__ partial, int nslab
     }
     for (; s < nslab; ++s) a += partial[(size_t)s * count + i];
     out[i] = a;
+}
+
+// First pass of a two-pass slab reduction: part p of P sums slabs
+// [p*nslab/P, (p+1)*nslab/P) in order into tmp[p][i], so P times more threads
+// share the reads; reduce_slabs_kernel then adds the P parts in order (a fixed
+// tree: the same bits on every call).
+__global__ void reduce_slab_parts_kernel(const float *__restrict__ partial, int nslab, size_t count, int nparts,
+                                         float *__restrict__ tmp) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const int part = blockIdx.y;
+    if (i >= count) return;
+    const int s0 = (int)((long long)nslab * part / nparts), s1 = (int)((long long)nslab * (part + 1) / nparts);
+    float a = 0.f;
+    int s = s0;
+    for (; s + 8 <= s1; s += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = partial[(size_t)(s + j) * count + i];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a += v[j];
+    }
+    for (; s < s1; ++s) a += partial[(size_t)s * count + i];
+    tmp[(size_t)part * count + i] = a;
 }
 
 // per-slab channel sums of dZ (bias gradient), partial[slab][c].  256 threads
@@ -585,6 +762,8 @@ constexpr int kHeadIn = 16;
 
 inline int pad32(int k) { return (k + 31) / 32 * 32; }
 
+constexpr int kReduceParts = 16;  // parts of the first pass of the slab reductions
+
 size_t param_count() {
     size_t c = 0;
     for (const auto &s : kConv) c += (size_t)s.cout * s.cin_real * 27 + s.cout;
@@ -606,6 +785,7 @@ struct cfm_unet {
     bf16 *Wf[kNConv] = {}, *Wt[kNConv] = {};
     int Kp[kNConv], Kpt[kNConv];
     float *gk = nullptr;  // weight gradient in kernel layout (largest layer)
+    float *rtmp = nullptr;  // two-pass slab reduction: [kReduceParts][largest layer]
     float *partial = nullptr;
     size_t partial_cap = 0;
     int slab = 4096;
@@ -688,6 +868,7 @@ extern "C" cfm_status cfm_unet_create(int n, int batch, cfm_unet **out) {
         max_partial = std::max(max_partial, nslab * kConv[i].cout * u->Kp[i]);
     }
     A(&u->gk, 4 * max_wk);
+    A(&u->rtmp, 4 * kReduceParts * max_wk);
     A(&u->partial, 4 * max_partial);
     u->partial_cap = max_partial;
     const size_t P1 = u->P[0], P2 = u->P[1], P3 = u->P[2];
@@ -757,11 +938,12 @@ extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
 template <int MT, int CIN>
 static void launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
                             const float *bias, bf16 *Y, int relu) {
-    // tile 16 x TY x TZ points; the staged block stays within 64 KiB of LDS: up to 32 input
-    // channels 16 x 8 x 2 (four rows per wavefront), 48-64 channels 16 x 4 x 2, 96 channels 16 x 4 x 1
-    constexpr int TY = CIN <= 32 ? 8 : 4;
+    // tile 16 x TY x TZ points, the staged block at most 80 KiB of LDS (two workgroups per CU): up
+    // to 48 input channels 16 x 8 x 2 (four rows per wavefront), 64 channels 16 x 4 x 2, 96
+    // channels 16 x 4 x 1
+    constexpr int TY = CIN <= 48 ? 8 : 4;
     constexpr int TZ = CIN <= 64 ? 2 : 1;
-    static_assert((TZ + 2) * (TY + 2) * 18 * CIN * 2 <= 65536, "staged block above 64 KiB");
+    static_assert((TZ + 2) * (TY + 2) * 18 * CIN * 2 <= 80 * 1024, "staged block above 80 KiB");
     const int nbx = (n + 15) / 16, nby = (n + TY - 1) / TY, nbz = (n + TZ - 1) / TZ;
     const dim3 grid((unsigned)(nbx * nby * nbz * batch));
     hipLaunchKernelGGL((conv3_fwd_lds_kernel<MT, CIN, TY, TZ>), grid, dim3(256), 0, st, X, W, kp, bias, Y, n, nbx, nby,
@@ -862,8 +1044,46 @@ extern "C" cfm_status cfm_unet_forward(cfm_unet *u, const float *input, float *o
 // the LDS-staged weight gradient for the network's (input, output) channel pairs
 // with 16-multiple input channels; the first layer (8 padded input channels)
 // keeps the gather kernel
+template <int CIN, int COUT, int WROWS, int KS = 1>
+static void launch_wgrad_rows(int nslab, hipStream_t st, const bf16 *X, const bf16 *dZ, int kp, float *partial, int n,
+                              int npts, int slab) {
+    constexpr int CINP = CIN / KS < 16 ? 16 : CIN / KS;
+    const int R = 64 / n;
+    const size_t lds = sizeof(bf16) * ((size_t)COUT * 72 + (size_t)3 * (R + 2) * CINP * (n + 8));
+    // (the largest staging, 48 input channels at n = 64, is just above 64 KiB)
+    static const bool big_lds_ok =
+        hipFuncSetAttribute((const void *)conv3_wgrad_rows_kernel<CIN, COUT, WROWS, KS>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+    (void)big_lds_ok;
+    hipLaunchKernelGGL((conv3_wgrad_rows_kernel<CIN, COUT, WROWS, KS>), dim3(nslab, KS), dim3(64 * 9 / WROWS), lds,
+                       st, X, dZ, kp, partial, n, npts, slab);
+}
+
+// every tap in one workgroup (whole x rows per 64-point block): n = 16, 32, 64 and slabs of
+// whole blocks; three (dz, dy) rows per wavefront up to 4 channel-tile pairs, else one
+static bool wgrad_rows(int cin, int cout, int nslab, hipStream_t st, const bf16 *X, const bf16 *dZ, int kp,
+                       float *partial, int n, int npts, int slab) {
+    if ((n != 16 && n != 32 && n != 64) || slab % 64 != 0 || npts % 64 != 0) return false;
+#define CFM_WGRAD_ROWS(C, O, WR, KS)                                                    \
+    if (cin == C && cout == O) {                                                        \
+        launch_wgrad_rows<C, O, WR, KS>(nslab, st, X, dZ, kp, partial, n, npts, slab);  \
+        return true;                                                                    \
+    }
+    CFM_WGRAD_ROWS(8, 16, 3, 1)
+    CFM_WGRAD_ROWS(16, 16, 3, 1)
+    CFM_WGRAD_ROWS(16, 32, 3, 1)
+    CFM_WGRAD_ROWS(32, 32, 3, 1)
+    CFM_WGRAD_ROWS(48, 16, 3, 1)
+    CFM_WGRAD_ROWS(32, 64, 1, 1)
+    CFM_WGRAD_ROWS(64, 64, 1, 2)  // (the input channels split over two workgroups: the accumulators
+    CFM_WGRAD_ROWS(96, 32, 1, 2)  //  of all of them do not fit in registers)
+#undef CFM_WGRAD_ROWS
+    return false;
+}
+
 static bool wgrad_lds(int cin, int cout, int nslab, hipStream_t st, const bf16 *X, const bf16 *dZ, int kp,
                       float *partial, int n, int npts, int slab) {
+    if (wgrad_rows(cin, cout, nslab, st, X, dZ, kp, partial, n, npts, slab)) return true;
     const dim3 grid(27, nslab);
 #define CFM_WGRAD_LDS(C, O)                                                                              \
     if (cin == C && cout == O) {                                                                         \
@@ -882,6 +1102,20 @@ static bool wgrad_lds(int cin, int cout, int nslab, hipStream_t st, const bf16 *
     return false;
 }
 
+// out[i] = sum over the nslab slab partials of u->partial (count values each), in two passes
+static void reduce_slabs(cfm_unet *u, int nslab, size_t count, float *out) {
+    if (nslab < 4 * kReduceParts) {
+        hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->partial, nslab, count,
+                           out);
+        return;
+    }
+    const dim3 grid((unsigned)((count + 255) / 256), kReduceParts);
+    hipLaunchKernelGGL(reduce_slab_parts_kernel, grid, dim3(256), 0, u->stream, u->partial, nslab, count, kReduceParts,
+                       u->rtmp);
+    hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->rtmp, kReduceParts, count,
+                       out);
+}
+
 static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) {
     const ConvSpec &s = kConv[I];
     const int level = s.level, n = u->n >> level;
@@ -895,13 +1129,12 @@ static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) 
         hipLaunchKernelGGL(conv3_wgrad_kernel, dim3((ntile + 3) / 4, nslab), dim3(256), 0, u->stream, X, s.cin, dZ,
                            s.cout, u->Kp[I], u->partial, n, (int)npts, (int)slab, ntile_k);
     CFM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(count), dim3(256), 0, u->stream, u->partial, nslab, count, u->gk);
+    reduce_slabs(u, nslab, count, u->gk);
     hipLaunchKernelGGL(unpack_wgrad_kernel, blocks_for((size_t)s.cout * s.cin_real * 27), dim3(256), 0, u->stream,
                        u->gk, s.cout, s.cin_real, s.cin, u->Kp[I], u->grad + u->woff[I]);
     hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(256), 0, u->stream, dZ, s.cout, (int)npts, (int)slab,
                        u->partial);
-    hipLaunchKernelGGL(reduce_slabs_kernel, blocks_for(s.cout), dim3(256), 0, u->stream, u->partial, nslab,
-                       (size_t)s.cout, u->grad + u->boff[I]);
+    reduce_slabs(u, nslab, (size_t)s.cout, u->grad + u->boff[I]);
     CFM_HIP(hipGetLastError());
     return CFM_OK;
 }

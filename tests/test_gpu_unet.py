@@ -36,7 +36,9 @@ def test_forward_matches_oracle(cfm, gpu, n, batch):
     assert _rel(out.reshape(batch, -1), ref) < 5e-3
 
 
-@pytest.mark.parametrize("n,batch", [(16, 2), (12, 3), (8, 1)])
+# 64 and 32: the training size's grids, where every level takes the all-taps weight-gradient
+# kernel (whole x rows per 64-point block) and the taller forward tiles
+@pytest.mark.parametrize("n,batch", [(16, 2), (12, 3), (8, 1), (32, 1), (64, 1)])
 def test_gradient_matches_oracle(cfm, gpu, n, batch):
     params, inp, target = _case(20 + n, n, batch)
     with cfm.UNet(n, batch, params=params) as net:

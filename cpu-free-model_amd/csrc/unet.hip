@@ -98,13 +98,16 @@ __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restri
         ty[t] = (w * NT + t) % TY;
         tz[t] = (w * NT + t) / TY;
     }
-    constexpr int K = 27 * CIN;
+    constexpr int K = 27 * CIN, KPC = (K + 31) / 32 * 32;  // Kp == KPC (checked by the host)
     f32x4 acc[MT][NT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < Kp; kb += 32) {
+    // a compile-time trip count, unrolled by eight: the weight loads of eight k steps go out
+    // together instead of one L2 round trip per k step
+#pragma unroll 8
+    for (int kb = 0; kb < KPC; kb += 32) {
         const int k = kb + 8 * g;
         bf16x8 av[MT];
 #pragma unroll
@@ -936,8 +939,9 @@ extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
 }
 
 template <int MT, int CIN>
-static void launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
+static bool launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
                             const float *bias, bf16 *Y, int relu) {
+    if (kp != (27 * CIN + 31) / 32 * 32) return false;
     // tile 16 x TY x TZ points, the staged block at most 80 KiB of LDS (two workgroups per CU): up
     // to 48 input channels 16 x 8 x 2 (four rows per wavefront), 64 channels 16 x 4 x 2, 96
     // channels 16 x 4 x 1
@@ -948,6 +952,7 @@ static void launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, con
     const dim3 grid((unsigned)(nbx * nby * nbz * batch));
     hipLaunchKernelGGL((conv3_fwd_lds_kernel<MT, CIN, TY, TZ>), grid, dim3(256), 0, st, X, W, kp, bias, Y, n, nbx, nby,
                        nbz, relu);
+    return true;
 }
 
 // the (input channels, output channel tiles) pairs of the network's forward and data-gradient convs
@@ -955,8 +960,7 @@ static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf
                      const float *bias, bf16 *Y, int relu) {
 #define CFM_CONV_LDS(C, M)                                                  \
     if (cin == C && mt == M) {                                              \
-        launch_conv_lds<M, C>(n, batch, st, X, W, kp, bias, Y, relu);       \
-        return true;                                                        \
+        return launch_conv_lds<M, C>(n, batch, st, X, W, kp, bias, Y, relu); \
     }
     CFM_CONV_LDS(8, 1)
     CFM_CONV_LDS(16, 1)

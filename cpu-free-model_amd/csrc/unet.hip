@@ -146,6 +146,143 @@ __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restri
     }
 }
 
+// Forward conv with the three x taps in three accumulators, for rows of at
+// most 64 points.  A workgroup owns whole x rows (NXT 16-point tiles, zeros
+// beyond n) of TY x TZ (y, z) rows of one batch element and stages their
+// (TY+2) x (TZ+2) input rows, all CIN channels, in LDS; no x halo is needed.
+// K runs over the nine (dz, dy) taps and the channels, k = (3 dz + dy) CIN + ci
+// (a lane's 8 values: 8 channels of one tap).  Per 32-wide k block a lane reads
+// ONE input fragment, X at its own point x, rows y + dy, z + dz, and feeds it to
+// the three dx taps with their own weight fragments W[co][(3 (3 dz + dy) + dx) CIN
+// + ci]: acc[dx] at x collects the terms of X[x] that belong to output
+// Y[x + 1 - dx].  After the k loop Y[x] = acc[1][x] + acc[0][x - 1] + acc[2][x + 1]
+// (zero past the row ends: the padding there is zero): a DPP row shift per value,
+// the lane across a 16-point tile edge taken from the neighbouring tile by a row
+// rotation.  Per input fragment read from LDS the wavefront issues 3 MT MFMAs,
+// against MT when the x taps are separate k values.  RPW rows per wavefront; the
+// k loop unrolled by KU.
+template <int MT, int CIN, int TY, int TZ, int NXT, int NW, int KU>
+__global__ void __launch_bounds__(64 * NW) conv3_fwd_xacc_kernel(const bf16 *__restrict__ X,
+                                                                 const bf16 *__restrict__ W, int Kp,
+                                                                 const float *__restrict__ bias,
+                                                                 bf16 *__restrict__ Y, int n, int nby, int nbz,
+                                                                 int relu) {
+    constexpr int XW = NXT * 16, HY = TY + 2, HZ = TZ + 2, NV = CIN / 8;
+    constexpr int RPW = TY * TZ / NW;
+    constexpr int K9 = 9 * CIN, KB = (K9 + 31) / 32;
+    static_assert(CIN % 8 == 0 && (TY * TZ) % NW == 0, "shape");
+    __shared__ bf16x8 blk[HZ * HY * XW * NV];
+    int bid = blockIdx.x;
+    const int by = bid % nby;
+    bid /= nby;
+    const int bz = bid % nbz;
+    const int b = bid / nbz;
+    const int y0 = by * TY, z0 = bz * TZ;
+    const bf16x8 zero8 = {(bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f,
+                          (bf16)0.0f, (bf16)0.0f, (bf16)0.0f, (bf16)0.0f};
+    for (int v = threadIdx.x; v < HZ * HY * XW * NV; v += 64 * NW) {
+        const int cv = v % NV;
+        int r = v / NV;
+        const int cx = r % XW;
+        r /= XW;
+        const int cy = r % HY, cz = r / HY;
+        const int gy = y0 - 1 + cy, gz = z0 - 1 + cz;
+        bf16x8 val = zero8;
+        if (cx < n && gy >= 0 && gy < n && gz >= 0 && gz < n)
+            val = *reinterpret_cast<const bf16x8 *>(X + ((((size_t)b * n + gz) * n + gy) * n + cx) * CIN + cv * 8);
+        blk[v] = val;
+    }
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int nl = lane & 15, g = lane >> 4;
+    int ry[RPW], rz[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        ry[r] = (w * RPW + r) % TY;
+        rz[r] = (w * RPW + r) / TY;
+    }
+    f32x4 acc[3][MT][RPW][NXT];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int t = 0; t < NXT; ++t) acc[d][m][r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16 *wrow = W + (size_t)nl * Kp;
+#pragma unroll KU
+    for (int kb = 0; kb < KB; ++kb) {
+        const int k = kb * 32 + 8 * g;
+        // (K9 not a multiple of 32: the tail lanes read tap 8's fragment, then zero it)
+        const bool kin = K9 % 32 == 0 || k < K9;
+        const int dzdy = kin ? k / CIN : 8;
+        const int ci = kin ? k % CIN : 0;
+        const int dz = dzdy / 3, dy = dzdy % 3;
+        bf16x8 av[3][MT];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                av[d][m] = *reinterpret_cast<const bf16x8 *>(wrow + (size_t)m * 16 * Kp + (dzdy * 3 + d) * CIN + ci);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int t = 0; t < NXT; ++t) {
+                bf16x8 bv = blk[(((rz[r] + dz) * HY + (ry[r] + dy)) * XW + t * 16 + nl) * NV + ci / 8];
+                if (!kin) bv = zero8;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int m = 0; m < MT; ++m)
+                        acc[d][m][r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[d][m], bv, acc[d][m][r][t], 0, 0, 0);
+            }
+    }
+    // D[4g+i][nl] of tile t: output channel m*16 + 4g + i at x = 16 t + nl
+    constexpr int Cout = MT * 16;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int y = y0 + ry[r], z = z0 + rz[r];
+#pragma unroll
+        for (int t = 0; t < NXT; ++t) {
+            const int x = t * 16 + nl;
+            const bool out = x < n && y < n && z < n;
+            const size_t p = (((size_t)b * n + z) * n + y) * n + x;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                bf16x4 o;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    // (each element copied to a float first: a bit cast of a vector element
+                    // subscript reads element 0 in this compiler)
+                    const float c0 = acc[0][m][r][t][i], c2 = acc[2][m][r][t][i];
+                    const float p0 = t > 0 ? acc[0][m][r][t > 0 ? t - 1 : 0][i] : 0.f;
+                    const float q2 = t + 1 < NXT ? acc[2][m][r][t + 1 < NXT ? t + 1 : t][i] : 0.f;
+                    // acc[0] one point to the left: lane nl-1 of this tile (row_shr:1), lane 15 of
+                    // tile t-1 for nl = 0 (row_ror:1 brings lane 15 of each 16-lane row to lane 0)
+                    const int l0 = t > 0 ? __builtin_amdgcn_mov_dpp(__float_as_int(p0), 0x121 /* row_ror:1 */, 0xf, 0xf,
+                                                                    false)
+                                         : 0;
+                    const int s0 = __builtin_amdgcn_update_dpp(l0, __float_as_int(c0), 0x111 /* row_shr:1 */, 0xf,
+                                                               0xf, false);
+                    // acc[2] one point to the right: lane nl+1 (row_shl:1), lane 0 of tile t+1 for
+                    // nl = 15 (row_ror:15 brings lane 0 to lane 15)
+                    const int l2 = t + 1 < NXT ? __builtin_amdgcn_mov_dpp(__float_as_int(q2), 0x12f /* row_ror:15 */,
+                                                                          0xf, 0xf, false)
+                                               : 0;
+                    const int s2 = __builtin_amdgcn_update_dpp(l2, __float_as_int(c2), 0x101 /* row_shl:1 */, 0xf,
+                                                               0xf, false);
+                    float v = acc[1][m][r][t][i] + __int_as_float(s0) + __int_as_float(s2);
+                    v += bias ? bias[m * 16 + 4 * g + i] : 0.f;
+                    if (relu && v < 0.f) v = 0.f;
+                    o[i] = f2bf(v);
+                }
+                if (out) *reinterpret_cast<bf16x4 *>(Y + p * Cout + m * 16 + 4 * g) = o;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------ conv weight grad ---
 // partial[slab][co][k] = sum over points of slab: dZ[p][co] * Xs[p][k]
 // (M = 16 output channels, N = 16 k, K = 32 points per MFMA).  Each
@@ -492,7 +629,36 @@ __global__ void reduce_slab_parts_kernel(const float *__restrict__ partial, int 
     tmp[(size_t)part * count + i] = a;
 }
 
-// per-slab channel sums of dZ (bias gradient), partial[slab][c].  256 threads
+// out[r] = sum of row r of part ([rows][nslab], row-major), one 1024-thread
+// workgroup per row: thread t adds s = t, t + 1024, ... in order (eight loads in
+// flight), then a fixed pairwise tree over the threads, so every call gives the
+// same bits.  For the few-value sums (bias and head gradients) whose slab
+// counts run to tens of thousands.
+__global__ void __launch_bounds__(1024) reduce_rows_kernel(const float *__restrict__ part, int nslab,
+                                                           float *__restrict__ out) {
+    __shared__ float sh[1024];
+    const float *row = part + (size_t)blockIdx.x * nslab;
+    const int t = threadIdx.x;
+    float a = 0.f;
+    int s = t;
+    for (; s + 7 * 1024 < nslab; s += 8 * 1024) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = row[s + j * 1024];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a += v[j];
+    }
+    for (; s < nslab; s += 1024) a += row[s];
+    sh[t] = a;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[blockIdx.x] = sh[0];
+}
+
+// per-slab channel sums of dZ (bias gradient), partial[c][slab] (rows for reduce_rows_kernel).  256 threads
 // per slab: thread t < R*C (R = 256 / C rows at a time) owns channel t % C and
 // rows t / C, t / C + R, ... of the slab, so each pass reads 256 consecutive
 // values; the R row-partials of a channel are then added in row order.
@@ -520,7 +686,7 @@ __global__ void __launch_bounds__(256) chan_sum_kernel(const bf16 *__restrict__ 
     if (t < C) {
         float v = 0.f;
         for (int r = 0; r < R; ++r) v += red[r * C + t];
-        partial[(size_t)s * C + t] = v;
+        partial[(size_t)t * gridDim.x + s] = v;
     }
 }
 
@@ -675,7 +841,8 @@ __global__ void mse_kernel(const float *__restrict__ pred, const float *__restri
     if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
 }
 
-// head backward: dX[p][c] = dpred[p] w[c] (BF16); per-block partial grads of w (16) and b (1)
+// head backward: dX[p][c] = dpred[p] w[c] (BF16); per-block partial grads of w (16) and b (1),
+// gpart[c][block] (rows for reduce_rows_kernel)
 __global__ void head_bwd_kernel(const bf16 *__restrict__ X, const float *__restrict__ w,
                                 const float *__restrict__ dpred, bf16 *__restrict__ dX, float *__restrict__ gpart,
                                 size_t npts) {
@@ -687,9 +854,15 @@ __global__ void head_bwd_kernel(const bf16 *__restrict__ X, const float *__restr
     if (p < npts) {
         const float d = dpred[p];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            dX[p * 16 + c] = f2bf(d * w[c]);
-            gw[c] = d * bf2f(X[p * 16 + c]);
+        for (int h = 0; h < 2; ++h) {
+            const bf16x8 x = ld8(X + p * 16 + 8 * h);
+            bf16x8 o;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                o[j] = f2bf(d * w[8 * h + j]);
+                gw[8 * h + j] = d * bf2f(x[j]);
+            }
+            st8(dX + p * 16 + 8 * h, o);
         }
         gw[16] = d;
     }
@@ -702,7 +875,7 @@ __global__ void head_bwd_kernel(const bf16 *__restrict__ X, const float *__restr
             for (int c = 0; c < 17; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x < 17) gpart[(size_t)blockIdx.x * 17 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 17) gpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
 }
 
 // Adam on FP32 master parameters
@@ -955,6 +1128,51 @@ static bool launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, con
     return true;
 }
 
+#ifndef CFM_UNET_XACC
+#define CFM_UNET_XACC 1
+#endif
+
+// Rows of NXT 16-point tiles: eight (y, z) rows (4 x 2), two per wavefront, while the
+// accumulators (3 MT x 2 x NXT tiles) stay within 24 tiles, else 2 x 2 rows, one per
+// wavefront; eight wavefronts (one row each) when the eight-row block is too large for two
+// workgroups per CU.
+template <int MT, int CIN, int NXT>
+static void launch_conv_xacc(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
+                             const float *bias, bf16 *Y, int relu) {
+    constexpr bool two = MT * NXT <= 4;
+    constexpr int lds8 = NXT * 16 * 6 * 4 * CIN * 2;
+    constexpr int NW = two && lds8 > 80 * 1024 ? 8 : 4;
+    constexpr int TY = two ? 4 : 2, TZ = 2;
+    static_assert(NXT * 16 * (TY + 2) * (TZ + 2) * CIN * 2 <= 144 * 1024, "staged block above 144 KiB");
+    const int nby = (n + TY - 1) / TY, nbz = (n + TZ - 1) / TZ;
+    hipLaunchKernelGGL((conv3_fwd_xacc_kernel<MT, CIN, TY, TZ, NXT, NW, 2>), dim3((unsigned)(nby * nbz * batch)),
+                       dim3(64 * NW), 0, st, X, W, kp, bias, Y, n, nby, nbz, relu);
+}
+
+// The x-accumulator kernel where it measured faster than the per-tap one: the wide inputs of
+// the decoder, 48 -> 16 channels (rows of 64 points at 32 x 64^3: 1.06 vs 1.17 ms) and 96 -> 32
+// (rows of 32: 0.59 vs 0.93 ms).  With 8 to 32 input channels the per-tap kernel's smaller
+// staged block keeps 3 to 7 workgroups per CU against 1 to 3 and won (16 -> 16: 0.33 vs 0.41
+// ms), and the 16 -> 48 and 32 -> 96 data gradients need 3 x 3 and 3 x 6 accumulator tiles per
+// row: one wavefront per SIMD, twice the time.
+static bool conv_xacc(int cin, int mt, int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
+                      const float *bias, bf16 *Y, int relu) {
+    if (!CFM_UNET_XACC || n > 64 || kp < 27 * cin) return false;
+    const int nxt = n <= 16 ? 1 : n <= 32 ? 2 : 4;
+#define CFM_CONV_XACC(C, M, T)                                               \
+    if (cin == C && mt == M && nxt == T) {                                   \
+        launch_conv_xacc<M, C, T>(n, batch, st, X, W, kp, bias, Y, relu);    \
+        return true;                                                         \
+    }
+    CFM_CONV_XACC(48, 1, 4)
+    CFM_CONV_XACC(48, 1, 2)
+    CFM_CONV_XACC(48, 1, 1)
+    CFM_CONV_XACC(96, 2, 2)
+    CFM_CONV_XACC(96, 2, 1)
+#undef CFM_CONV_XACC
+    return false;
+}
+
 // the (input channels, output channel tiles) pairs of the network's forward and data-gradient convs
 static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
                      const float *bias, bf16 *Y, int relu) {
@@ -982,7 +1200,8 @@ static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf
 static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, int kp, const float *bias, bf16 *Y,
                            int cout, int level, int relu) {
     const int n = u->n >> level;
-    if (cout % 16 != 0 || !conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu))
+    if (cout % 16 != 0 || !(conv_xacc(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu) ||
+                            conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu)))
         return CFM_FAIL(CFM_EINVAL, "unet: no convolution kernel for %d -> %d channels", cin, cout);
     CFM_HIP(hipGetLastError());
     return CFM_OK;
@@ -1138,7 +1357,8 @@ static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) 
                        u->gk, s.cout, s.cin_real, s.cin, u->Kp[I], u->grad + u->woff[I]);
     hipLaunchKernelGGL(chan_sum_kernel, dim3(nslab), dim3(256), 0, u->stream, dZ, s.cout, (int)npts, (int)slab,
                        u->partial);
-    reduce_slabs(u, nslab, (size_t)s.cout, u->grad + u->boff[I]);
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(s.cout), dim3(1024), 0, u->stream, u->partial, nslab,
+                       u->grad + u->boff[I]);
     CFM_HIP(hipGetLastError());
     return CFM_OK;
 }
@@ -1165,7 +1385,7 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     hipLaunchKernelGGL(head_bwd_kernel, dim3(nb1), dim3(256), 0, u->stream, u->A9, hw, u->dpred, u->g1a, u->hpart, P1);
     CFM_HIP(hipGetLastError());
     // head gradient: reduce per-block partials (fixed order)
-    hipLaunchKernelGGL(reduce_slabs_kernel, dim3(1), dim3(64), 0, u->stream, u->hpart, (int)nb1, (size_t)17,
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(17), dim3(1024), 0, u->stream, u->hpart, (int)nb1,
                        u->grad + u->head_off);
     cfm_status st;
 #define CK(x) \

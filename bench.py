@@ -237,6 +237,7 @@ def cg_batch_figures(cfm, nsys, lmin=32, lmax=96, seed=2024):
         plan.set_rhs(b)
         plan.set_x0(None)
         blk, grid = plan.tune(trial_iters=20)
+        fmt = plan.matrix_format()
         plan.solve()
         st, iters, stat, hist, t = plan.solve()
     batched_ms = t["kernel_ms"]
@@ -253,7 +254,8 @@ def cg_batch_figures(cfm, nsys, lmin=32, lmax=96, seed=2024):
     return {"systems": nsys, "rows": int(srp[-1]), "edge": [lmin, lmax], "status": cfm.status_str(st),
             "batched_ms_to_1e-8": round(batched_ms, 2), "plain_ms_to_1e-8": round(plain_ms, 2),
             "speedup": round(plain_ms / batched_ms, 3), "max_iters": int(iters.max()),
-            "mean_iters": round(float(iters.mean()), 1), "tuned_block": blk, "tuned_grid": grid}
+            "mean_iters": round(float(iters.mean()), 1), "tuned_block": blk, "tuned_grid": grid,
+            "tuned_format": fmt}
 
 
 def cg_batch_profile_run(cfm, nsys, lmin=32, lmax=96, seed=2024):

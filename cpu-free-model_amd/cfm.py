@@ -36,7 +36,7 @@ EXPORTS = (
     "cfm_cg_plan_create", "cfm_cg_plan_set_rhs", "cfm_cg_plan_set_x0", "cfm_cg_plan_solve",
     "cfm_cg_plan_get_x", "cfm_cg_plan_destroy", "cfm_cg_solve", "cfm_gen_poisson3d", "cfm_gen_uniform",
     "cfm_gen_varcoef3d", "cfm_cg_batch_plan_create", "cfm_cg_batch_set_rhs", "cfm_cg_batch_set_x0",
-    "cfm_cg_batch_solve", "cfm_cg_batch_get_x", "cfm_cg_batch_tune", "cfm_cg_batch_plan_destroy",
+    "cfm_cg_batch_solve", "cfm_cg_batch_get_x", "cfm_cg_batch_tune", "cfm_cg_batch_matrix_format", "cfm_cg_batch_plan_destroy",
     "cfm_unet_param_count", "cfm_unet_create", "cfm_unet_destroy", "cfm_unet_init_params", "cfm_unet_set_params",
     "cfm_unet_get_params", "cfm_unet_forward", "cfm_unet_gradient", "cfm_unet_train_step",
     "cfm_cg_batch_x0_rejected", "cfm_cg_batch_load_features", "cfm_cg_batch_warm_start", "cfm_gen_varcoef3d_coef",
@@ -157,6 +157,8 @@ def _load():
     L.cfm_cg_batch_solve.argtypes = [_vp, _ip, _ip, _dp, ctypes.c_int, ctypes.POINTER(Timing)]
     L.cfm_cg_batch_get_x.argtypes = [_vp, _dp]
     L.cfm_cg_batch_tune.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, _ip, _ip]
+    L.cfm_cg_batch_matrix_format.argtypes = [_vp]
+    L.cfm_cg_batch_matrix_format.restype = ctypes.c_int
     L.cfm_cg_batch_plan_destroy.argtypes = [_vp]
     L.cfm_cg_batch_plan_destroy.restype = None
     _fp = ctypes.POINTER(ctypes.c_float)
@@ -690,6 +692,10 @@ class CgBatchPlan:
         _check(lib.cfm_cg_batch_tune(self._h, trial_iters, cache_path.encode() if cache_path else None,
                                      ctypes.byref(bb), ctypes.byref(bg)), "cfm_cg_batch_tune")
         return bb.value, bg.value
+
+    def matrix_format(self):
+        """Storage of the next solve: "sell" or "dia"."""
+        return {1: "sell", 2: "dia"}[lib.cfm_cg_batch_matrix_format(self._h)]
 
     def close(self):
         if self._h:

@@ -90,12 +90,13 @@ extern "C" cfm_status cfm_bandwidth_probe(size_t bytes, int reps, double *copy_g
         st = CFM_EDEVICE;
     } else {
         for (int r = 0; r < reps + 1 && st == CFM_OK; ++r) {
-            hipEventRecord(e0, nullptr);
+            const hipError_t r0 = hipEventRecord(e0, nullptr);
             hipLaunchKernelGGL(copy_probe_kernel, dim3(1024), dim3(256), 0, nullptr,
                                static_cast<const v2f64_t *>(a), static_cast<v2f64_t *>(b), n);
-            hipEventRecord(e1, nullptr);
+            const hipError_t r1 = hipEventRecord(e1, nullptr);
             float ms = 0.f;
-            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            if (r0 != hipSuccess || r1 != hipSuccess || hipGetLastError() != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
                 st = CFM_EDEVICE;
             else if (r > 0 && ms < best)
                 best = ms;

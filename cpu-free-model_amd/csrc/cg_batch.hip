@@ -490,9 +490,10 @@ struct cfm_cg_batch_plan {
     double *w[2] = {nullptr, nullptr};
     double *part = nullptr, *st_alpha = nullptr, *st_delta = nullptr, *hist = nullptr;
     int *fin = nullptr, *status = nullptr, *iters = nullptr, *x0_rejected = nullptr;
-    double *dval = nullptr;  // DIA storage (dmax > 0)
+    double *dval = nullptr;  // DIA storage (dmax > 0 when built)
     int *doff = nullptr;
     int dmax = 0;
+    bool dia_on = false;  // solve on DIA (else SELL-64); both are built for an auto plan of banded systems
     float *features = nullptr;  // warm-start network input [nsys][2][rows], loaded on demand
     int hist_cap = 0;
     GridSync *sync = nullptr;
@@ -642,24 +643,27 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
     pl->total_rows = sys_row_ptr[nsys];
     const long long padded_rows = chunks * kChunk;
     if (p->matrix_format < 0 || p->matrix_format > 2) return fail(CFM_FAIL(CFM_EINVAL, "batched CG: matrix_format"));
-    // DIA only when asked (every system must be banded): for batches it measured
-    // slower than SELL-64 (1301 vs 1133 ms for the 256-system 32-96 workload), its
-    // per-diagonal loop exposing load latency that the 8-wide SELL loop hides
+    // DIA when asked (every system must be banded), and beside SELL-64 for an auto
+    // plan of banded systems, where the autotuner picks one; solves start on SELL-64
+    // (for the 256-system 32-96 workload DIA measured 1301 vs 1133 ms, its per-diagonal
+    // loop exposing load latency that the 8-wide SELL loop hides)
     std::vector<int> doffs;
     std::vector<double> dvals;
     bool dia = false;
-    if (p->matrix_format == 2) {
+    if (p->matrix_format != 1) {
         dia = batch_dia(nsys, sys_row_ptr, rowptr, col, val, pl->h_chunk0, padded_rows, pl->dmax, doffs, dvals);
         if (!dia && p->matrix_format == 2)
             return fail(CFM_FAIL(CFM_ENOTSUP, "batched CG: DIA storage needs every system banded (<= %d diagonals, "
                                               "sorted rows)", kMaxDiagBatch));
         if (!dia) pl->dmax = 0;
     }
+    pl->dia_on = p->matrix_format == 2;
+    const bool sell = p->matrix_format != 2;
     // SELL-64 over padded rows; columns remapped, must stay inside the system
-    const long long nslices = dia ? 0 : padded_rows / 64;
+    const long long nslices = sell ? padded_rows / 64 : 0;
     std::vector<int> sp(nslices + 1, 0);
     long long tot = 0;
-    for (int sy = 0; sy < nsys && !dia; ++sy) {
+    for (int sy = 0; sy < nsys && sell; ++sy) {
         const long long g0 = sys_row_ptr[sy];
         const int nr = pl->h_rows[sy];
         const long long s0 = (long long)pl->h_chunk0[sy] * kSlicesPerChunk;
@@ -678,7 +682,7 @@ extern "C" cfm_status cfm_cg_batch_plan_create(int nsys, const int64_t *sys_row_
     sp[nslices] = (int)tot;
     std::vector<int> sc((size_t)tot, -1);
     std::vector<double> sv((size_t)tot, 0.0);
-    for (int sy = 0; sy < nsys && !dia; ++sy) {
+    for (int sy = 0; sy < nsys && sell; ++sy) {
         const long long g0 = sys_row_ptr[sy];
         const long long g1 = sys_row_ptr[sy + 1];
         const long long prow0 = (long long)pl->h_chunk0[sy] * kChunk;
@@ -806,7 +810,7 @@ static cfm_status batch_run(cfm_cg_batch_plan *pl, int max_iter, double tol, int
     A.x0_rejected = pl->x0_rejected;
     A.dval = pl->dval;
     A.doff = pl->doff;
-    A.dmax = pl->dmax;
+    A.dmax = pl->dia_on ? pl->dmax : 0;
     A.dpad = (long long)pl->total_chunks * kChunk;
     A.sync = pl->sync;
     CFM_HIP(hipMemsetAsync(pl->status, 0, sizeof(int) * pl->nsys, pl->stream));
@@ -873,68 +877,108 @@ extern "C" cfm_status cfm_cg_batch_solve(cfm_cg_batch_plan *pl, int *iters, int 
     return overall;
 }
 
-// Autotuner: time `trial_iters` fixed iterations of the batch for each
-// workgroup shape (1, 2 or 4 sub-blocks of 256 threads) and each grid size
-// (1..occupancy workgroups per CU); keep the fastest.  With a cache file the
-// winner is stored under a key of device, system count and row count, and
-// looked up first.
+// Autotuner.  Screen: time `trial_iters` fixed iterations of the batch for each
+// workgroup shape (1, 2 or 4 sub-blocks of 256 threads), each grid size (1..occupancy
+// workgroups per CU) and each storage the plan holds (SELL-64, and DIA when an auto
+// plan's systems are banded).  Then time full solves (the plan's tol and max_iter,
+// from the current x0) of the kTuneFinalists fastest and keep the fastest solve: the
+// first iterations, where every system is active, do not show the tail, where the
+// few systems still iterating leave most of the grid waiting at barriers.  With a
+// cache file the winner is stored under a key of device, system count, row count and
+// tolerance, and looked up first.
+constexpr int kTuneFinalists = 3;
+
 extern "C" cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *pl, int trial_iters, const char *cache_path,
                                         int *best_block, int *best_grid) {
     if (!pl || trial_iters < 1) return CFM_EINVAL;
     char key[512];
-    std::snprintf(key, sizeof(key), "cg_batch|%s|cus=%d|nsys=%d|rows=%lld|chunks=%d", pl->dev.name, pl->dev.cus,
-                  pl->nsys, pl->total_rows, pl->total_chunks);
+    std::snprintf(key, sizeof(key), "cg_batch|%s|cus=%d|nsys=%d|rows=%lld|chunks=%d|tol=%.3g", pl->dev.name,
+                  pl->dev.cus, pl->nsys, pl->total_rows, pl->total_chunks, pl->p.tol);
     for (char *q = key; *q; ++q)  // the cache line is whitespace-separated
         if (*q == ' ' || *q == '\t') *q = '_';
+    const bool have_sell = pl->p.matrix_format != 2, have_dia = pl->dmax > 0;
+    auto use = [&](int sub, int grid, bool dia) {
+        if ((dia && !have_dia) || (!dia && !have_sell)) return CFM_EINVAL;
+        const cfm_status st = batch_configure(pl, sub, grid);
+        if (st == CFM_OK) pl->dia_on = dia;
+        return st;
+    };
+    auto report = [&]() {
+        if (best_block) *best_block = 256 * pl->sub;
+        if (best_grid) *best_grid = pl->grid;
+    };
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "r")) {
             char line[1024];
             while (std::fgets(line, sizeof(line), f)) {
                 char k2[600];
-                int s2 = 0, g2 = 0;
-                if (std::sscanf(line, "%599s %d %d", k2, &s2, &g2) == 3 && std::strcmp(k2, key) == 0 &&
-                    batch_configure(pl, s2, g2) == CFM_OK) {
+                int s2 = 0, g2 = 0, d2 = 0;
+                if (std::sscanf(line, "%599s %d %d %d", k2, &s2, &g2, &d2) == 4 && std::strcmp(k2, key) == 0 &&
+                    use(s2, g2, d2 != 0) == CFM_OK) {
                     std::fclose(f);
-                    if (best_block) *best_block = 256 * pl->sub;
-                    if (best_grid) *best_grid = pl->grid;
+                    report();
                     return CFM_OK;
                 }
             }
             std::fclose(f);
         }
     }
-    float best = 1e30f;
-    int bs = pl->sub, bg = pl->grid;
+    struct Cand {
+        int sub, grid;
+        bool dia;
+        float ms;
+    };
+    std::vector<Cand> cands;
     const int subs[] = {1, 2, 4};
-    for (int sub : subs) {
-        BatchKernelFn k = batch_kernel(sub);
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, 256 * sub, 0) != hipSuccess)
-            continue;
-        for (int m = 1; m <= per_cu; ++m) {
-            if (batch_configure(pl, sub, m * pl->dev.cus) != CFM_OK) continue;
-            float ms = 0.f;
-            int coop = -1;
-            cfm_status st = batch_run(pl, trial_iters, 0.0, 0, &ms, &coop);
-            if (st != CFM_OK) return st;
-            if (ms < best) {
-                best = ms;
-                bs = pl->sub;
-                bg = pl->grid;
+    for (int storage = 0; storage < 2; ++storage) {
+        const bool dia = storage == 1;
+        if ((dia && !have_dia) || (!dia && !have_sell)) continue;
+        for (int sub : subs) {
+            BatchKernelFn k = batch_kernel(sub);
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, 256 * sub, 0) != hipSuccess)
+                continue;
+            for (int m = 1; m <= per_cu; ++m) {
+                if (use(sub, m * pl->dev.cus, dia) != CFM_OK) continue;
+                float ms = 0.f;
+                int coop = -1;
+                cfm_status st = batch_run(pl, trial_iters, 0.0, 0, &ms, &coop);
+                if (st != CFM_OK) return st;
+                cands.push_back({pl->sub, pl->grid, dia, ms});
             }
         }
     }
-    cfm_status st = batch_configure(pl, bs, bg);
+    if (cands.empty()) return CFM_FAIL(CFM_EINVAL, "batched CG: no workgroup shape fits");
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    if (cands.size() > (size_t)kTuneFinalists) cands.resize(kTuneFinalists);
+    Cand best = cands[0];
+    float best_ms = 1e30f;
+    for (const Cand &c : cands) {
+        if (use(c.sub, c.grid, c.dia) != CFM_OK) continue;
+        float ms = 0.f;
+        int coop = -1;
+        cfm_status st = batch_run(pl, pl->p.max_iter, pl->p.tol, 0, &ms, &coop);
+        if (st != CFM_OK) return st;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    cfm_status st = use(best.sub, best.grid, best.dia);
     if (st != CFM_OK) return st;
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s %d %d\n", key, bs, bg);
+            std::fprintf(f, "%s %d %d %d\n", key, best.sub, best.grid, best.dia ? 1 : 0);
             std::fclose(f);
         }
     }
-    if (best_block) *best_block = 256 * bs;
-    if (best_grid) *best_grid = bg;
+    report();
     return CFM_OK;
+}
+
+extern "C" int cfm_cg_batch_matrix_format(const cfm_cg_batch_plan *pl) {
+    if (!pl) return -1;
+    return pl->dia_on ? 2 : 1;
 }
 
 extern "C" cfm_status cfm_cg_batch_x0_rejected(cfm_cg_batch_plan *pl, int *flags) {
@@ -1000,7 +1044,7 @@ extern "C" cfm_status cfm_cg_batch_smooth_x0(cfm_cg_batch_plan *pl, int sweeps, 
     CFM_HIP(hipEventRecord(pl->ev0, pl->stream));
     double *src = pl->x0, *dst = pl->r;  // r is scratch until the next solve initialises it
     for (int k = 0; k < sweeps; ++k) {
-        if (pl->dmax > 0)
+        if (pl->dia_on)
             hipLaunchKernelGGL(jacobi_smooth_dia_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0,
                                pl->stream, pl->dval, pl->doff, pl->dmax, (long long)nrows, pl->sys_chunk0,
                                pl->sys_rows, pl->nsys, pl->b, src, dst, nrows, omega);

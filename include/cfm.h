@@ -208,7 +208,9 @@ typedef struct {
                             ||b - A x0|| > ||b|| (the guard for learned initial guesses) */
     int matrix_format;   /* 0 = auto (single-system Pipelined with >= 2^20 rows: DIA when the CSR
                             is banded with <= 16 diagonals and sorted rows; otherwise SELL-64),
-                            1 = SELL-64, 2 = DIA (every system of a batch; ENOTSUP if not banded) */
+                            1 = SELL-64, 2 = DIA (every system of a batch; ENOTSUP if not banded);
+                            an auto batched plan holds SELL-64 and, when every system is banded,
+                            DIA beside it for the autotuner to choose (solves start on SELL-64) */
     int phase_timers;    /* nonzero: in-place time breakdown of the solve (cfm_cg_plan_phase_times):
                             the persistent kernel's workgroups record a clock at every phase boundary
                             (a separate kernel build: the default build has no timer code); the
@@ -267,11 +269,17 @@ cfm_status cfm_cg_batch_solve(cfm_cg_batch_plan *plan, int *iters, int *status, 
                               int hist_cap, cfm_timing *t);
 cfm_status cfm_cg_batch_get_x(cfm_cg_batch_plan *plan, double *x);
 /* Autotuner: times trial_iters fixed iterations for each workgroup shape
- * (256/512/1024 threads) and grid size (1..occupancy workgroups per CU) and
- * keeps the fastest; cache_path (may be NULL) stores/looks up the winner per
- * (device, system count, row count). */
+ * (256/512/1024 threads), grid size (1..occupancy workgroups per CU) and
+ * storage the plan holds (SELL-64; DIA too for an auto plan whose systems are
+ * all banded), then times full solves (the plan's tol and max_iter, from the
+ * current x0) of the three fastest and keeps the fastest solve; cache_path
+ * (may be NULL) stores/looks up the winner per (device, system count, row
+ * count, tolerance).  The reference has no tuner: its launch shapes are
+ * fixed per binary (CG/src/single-stream/pipelined.cu). */
 cfm_status cfm_cg_batch_tune(cfm_cg_batch_plan *plan, int trial_iters, const char *cache_path,
                              int *best_block, int *best_grid);
+/* Storage the next solve uses: 1 = SELL-64, 2 = DIA (-1 for a NULL plan). */
+int cfm_cg_batch_matrix_format(const cfm_cg_batch_plan *plan);
 void cfm_cg_batch_plan_destroy(cfm_cg_batch_plan *plan);
 /* After a solve with x0_guard set: flags[s] = 1 where system s was restarted
  * from x0 = 0 because its initial guess was worse than zero. */

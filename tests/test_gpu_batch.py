@@ -137,6 +137,8 @@ def test_batch_autotune_and_cache(gpu, orc, tmp_path):
         plan.set_x0(None)
         blk, grid = plan.tune(trial_iters=10, cache_path=cache)
         assert blk in (256, 512, 1024) and grid >= 1
+        fmt = plan.matrix_format()  # the 7-point systems are banded: SELL-64 and DIA both compete
+        assert fmt in ("sell", "dia")
         st, iters, stat, hist, t = plan.solve(hist_cap=3001)
         x = plan.x()
         assert t["block_threads"] == blk and t["grid_blocks"] == grid
@@ -144,7 +146,9 @@ def test_batch_autotune_and_cache(gpu, orc, tmp_path):
     with open(cache) as fh:
         assert len(fh.read().strip().splitlines()) == 1
     with gpu.CgBatchPlan(srp, rp, col, val, p) as plan:  # cache hit, no new line
+        assert plan.matrix_format() == "sell"  # solves start on SELL-64
         assert plan.tune(trial_iters=10, cache_path=cache) == (blk, grid)
+        assert plan.matrix_format() == fmt
     with open(cache) as fh:
         assert len(fh.read().strip().splitlines()) == 1
 

@@ -969,21 +969,37 @@ struct Slot {
     static constexpr int value = S;
 };
 
-template <typename T, int WX, int WY, int VXT, bool FULL, bool PER>
+//
+// XT: rows wider than one tile.  Tile tx covers columns X0 .. X0 + TW - 1 with
+// X0 = tx (TW - E) (16-byte aligned).  Level 1 is computed on all of them: the
+// neighbours of the two outer columns, X0 - 1 and X0 + TW, are level-0 values
+// that the edge lanes load themselves, one plane step ahead like the side rows.
+// Level 2 is then valid on X0 + 1 .. X0 + TW - 2, and the tile owns (stores and
+// adds to the norms) X0 + 1 .. X0 + TW - E, so consecutive tiles own consecutive
+// columns; each pass recomputes E + 1 level-1 columns per tile boundary.
+template <typename T, int WX, int WY, int VXT, bool FULL, bool PER, bool XT>
 __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
-                                            int ty, int zc, bool single, bool want0, bool want1, double &n0,
+                                            int tx, int ty, int zc, bool single, bool want0, bool want1, double &n0,
                                             double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
     using Tl = T2Tile<T, WX, WY, VXT>;
     constexpr int VX = Tl::VX;
     constexpr int E = 16 / sizeof(T);
     static_assert(WY >= 2, "the top and bottom wave must differ");
     static_assert(VX % E == 0, "whole 16-byte vectors");
+    static_assert(!(XT && FULL), "full-width rows are one tile");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: roles become scalar branches
     const int wx = wave % WX, wy = wave / WX;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const size_t plane = (size_t)nx * ny;
-    const int x = wx * Tl::SEG + lane * VX;
+    const int X0 = XT ? tx * (Tl::TW - E) : 0;
+    const int xl = wx * Tl::SEG + lane * VX;  // the lane's first column in the tile (LDS rows)
+    const int x = X0 + xl;                    // ... and in the field
+    // owned columns (stores, norms) and, for the periodic halo-plane copies, owned whole-row columns
+    const int olo = XT ? max(1, X0 + 1) : 1;
+    const int ohi = XT ? min(nx - 2, X0 + Tl::TW - E) : nx - 2;
+    const int clo = XT && X0 > 0 ? X0 + 1 : 0;
+    const int chi = XT && X0 + Tl::TW - E < nx - 2 ? X0 + Tl::TW - E : nx - 1;
     const int y = ty * WY + wy;
     const int zb = 1 + zc * A.chunk_len;
     const int ze = min(zb + A.chunk_len, nz - 1);
@@ -995,9 +1011,12 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     const bool has_h = top || bot;
     const int ys0 = top ? y - 2 : y - 1;
     const int ys1 = bot ? y + 2 : y + 1;
-    bool xin[VX];
+    bool xin[VX], own[VX];
 #pragma unroll
-    for (int k = 0; k < VX; ++k) xin[k] = x + k >= 1 && x + k <= nx - 2;
+    for (int k = 0; k < VX; ++k) {
+        xin[k] = x + k >= 1 && x + k <= nx - 2;
+        own[k] = XT ? x + k >= olo && x + k <= ohi : xin[k];
+    }
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(plane * nz * sizeof(T)), 0x00020000);
     constexpr unsigned kOut = 0x80000000u;  // beyond every field served here: the load returns zeros
@@ -1008,6 +1027,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     T rsd[2][VX];     // side rows of plane p+1 (in flight during a step)
     T l1o[3][VX];
     T l1h[2][VX];
+    T hx[2] = {T(0), T(0)}, hxn[2] = {T(0), T(0)};  // XT: level-0 values beside the tile (rows y, yc1)
 
     // a lane's column as a vector offset, the row as a scalar one; outside the field the vector
     // offset carries kOut and the scalar one is 0
@@ -1034,6 +1054,29 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         ldrow(z, ys0, true, v[0]);
         ldrow(z, ys1, true, v[1]);
     };
+    // XT: column X0 - 1 for lane 0 of the west wave, X0 + TW for lane 63 of the east wave (zero
+    // where that column lies outside the field: there the edge lane holds a boundary column,
+    // whose value is kept)
+    const unsigned hvo = (XT && wx == 0 && lane == 0 && X0 >= 1)                ? (unsigned)(X0 - 1) * sizeof(T)
+                         : (XT && wx == WX - 1 && lane == 63 && X0 + Tl::TW < nx) ? (unsigned)(X0 + Tl::TW) * sizeof(T)
+                                                                                   : kOut;
+    auto ld_halo = [&](int z, T (&h)[2]) {
+        if constexpr (XT) {
+#pragma unroll
+            for (int sl = 0; sl < 2; ++sl) {
+                const int yy = sl == 0 ? y : yc1;
+                const bool ok = (sl == 0 || has_h) && z >= 0 && z < nz && yy >= 0 && yy < ny;  // wave-uniform
+                const unsigned so = ok ? (unsigned)(((unsigned)z * (unsigned)ny + (unsigned)yy) * (unsigned)nx) *
+                                             (unsigned)sizeof(T)
+                                       : 0u;
+                const unsigned vo = ok ? hvo : kOut;
+                if constexpr (sizeof(T) == 8)
+                    h[sl] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)vo, (int)so, 0));
+                else
+                    h[sl] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)vo, (int)so, 0));
+            }
+        }
+    };
     // x edges of the level-0 rows of plane p+1: every lane stores one value into its own slot
     // (lane 0 its first column, the others their last; no masked stores), the neighbouring wave
     // reads the slot of lane 63 or lane 0 (a broadcast read)
@@ -1054,8 +1097,12 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     };
     auto l1row = [&](const typename Tl::Buf &B, int sl, int yy, bool pin, const T (&cr)[VX], const T (&N)[VX],
                      const T (&S)[VX], const T (&U)[VX], const T (&D)[VX], T (&out)[VX], bool norm) {
-        T west, east;
-        xnb(cr, B.l0x[wy][wx > 0 ? wx - 1 : 0][sl][63], B.l0x[wy][wx < WX - 1 ? wx + 1 : 0][sl][0], west, east);
+        T west, east, wn = B.l0x[wy][wx > 0 ? wx - 1 : 0][sl][63], en = B.l0x[wy][wx < WX - 1 ? wx + 1 : 0][sl][0];
+        if constexpr (XT) {  // the tile's own x ends take the loaded neighbours (wave-uniform)
+            if (wx == 0) wn = hx[sl];
+            if (wx == WX - 1) en = hx[sl];
+        }
+        xnb(cr, wn, en, west, east);
         if (pin && yy >= 1 && yy <= ny - 2) {  // wave-uniform
 #pragma unroll
             for (int k = 0; k < VX; ++k) {
@@ -1063,9 +1110,9 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
                 const T Wv = (k > 0) ? cr[k - 1] : west;
                 const T v = c * (Ev + Wv + N[k] + S[k] + U[k] + D[k]);
                 out[k] = (FULL && k > 0 && k < VX - 1) ? v : (xin[k] ? v : cr[k]);
-                if (norm) {  // wave-uniform; adding +0.0 outside the interior leaves the sum's bits alone
+                if (norm) {  // wave-uniform; adding +0.0 outside the owned columns leaves the sum's bits alone
                     const double d = (double)v - (double)cr[k];
-                    n0 += xin[k] ? d * d : 0.0;
+                    n0 += own[k] ? d * d : 0.0;
                 }
             }
         } else {
@@ -1088,7 +1135,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     // counted at 1.126 x the written field, and measured no faster)
     auto store_row = [&](int z, const T (&out)[VX]) {
         T *drow = dst + (size_t)z * plane + (size_t)y * nx;
-        if (FULL || (x >= 1 && x + VX - 1 <= nx - 2)) {
+        if (FULL || (x >= olo && x + VX - 1 <= ohi)) {
 #pragma unroll
             for (int q = 0; q < VX / E; ++q) {
                 typename Vec16<T>::type t;
@@ -1099,7 +1146,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         } else {
 #pragma unroll
             for (int k = 0; k < VX; ++k)
-                if (xin[k]) drow[x + k] = out[k];
+                if (own[k]) drow[x + k] = out[k];
         }
     };
     auto store_copy = [&](int z, const T (&out)[VX]) {
@@ -1116,7 +1163,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         } else {
 #pragma unroll
             for (int k = 0; k < VX; ++k)
-                if (x + k < nx) drow[x + k] = out[k];
+                if (x + k >= clo && x + k <= chi) drow[x + k] = out[k];
         }
     };
 
@@ -1128,6 +1175,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         ld_col(nz - 2, col[1]);
         ld_col(nz - 1, col[2]);
         ld_side(nz - 2, sid);
+        ld_halo(nz - 2, hx);
         put_l0e(lds[0], col[1]);
         __syncthreads();
         level1(lds[0], true, false);
@@ -1142,6 +1190,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     ld_col(p0, col[1]);
     ld_col(p0 + 1, col[2]);
     ld_side(p0, sid);
+    ld_halo(p0, hx);
     ld_col(p0 + 2, rc[1]);  // slot of step p0 + 1, moved into the window at the end of step p0
     put_l0e(lds[p0 % 3], col[1]);
     __syncthreads();
@@ -1159,11 +1208,13 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
             ld_col(1, col[1]);
             ld_col(2, col[2]);
             ld_side(1, sid);
+            ld_halo(1, hx);
             put_l0e(Bp, col[1]);
             __syncthreads();
             pin = true;
         } else {
             ld_side(p + 1, rsd);   // into the window at the end of this step (issued first)
+            ld_halo(p + 1, hxn);
             ld_col(p + 3, rc[S]);  // into the window at the end of step p + 1
         }
         const bool nrm = want0 && p >= zb && p < ze;
@@ -1178,7 +1229,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
             put_l0e(Bn, col[2]);
             __syncthreads();
         } else {
-            st_vec<T, VX>(&Bp.row[wy][x], l1o[2]);  // (its x edges are read from here too)
+            st_vec<T, VX>(&Bp.row[wy][xl], l1o[2]);  // (its x edges are read from here too)
             put_l0e(Bn, col[2]);
             __syncthreads();
             const int zo = p - 1;
@@ -1189,13 +1240,13 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
 #pragma unroll
                     for (int k = 0; k < VX; ++k) Sv[k] = l1h[0][k];
                 } else {
-                    ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x], Sv);
+                    ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][xl], Sv);
                 }
                 if (bot) {
 #pragma unroll
                     for (int k = 0; k < VX; ++k) N[k] = l1h[0][k];
                 } else {
-                    ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x], N);
+                    ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][xl], N);
                 }
                 const T (&cr)[VX] = l1o[1];
                 T west, east;
@@ -1210,7 +1261,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
                     out[k] = (FULL && k > 0 && k < VX - 1) ? v : (xin[k] ? v : cr[k]);
                     if (want1) {
                         const double d = (double)v - (double)cr[k];
-                        n1 += xin[k] ? d * d : 0.0;
+                        n1 += own[k] ? d * d : 0.0;
                     }
                 }
                 store_row(zo, out);
@@ -1238,6 +1289,10 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
             l1o[1][k] = l1o[2][k];
             l1h[0][k] = l1h[1][k];
         }
+        if constexpr (XT) {
+            hx[0] = hxn[0];
+            hx[1] = hxn[1];
+        }
     };
     if constexpr (PER) {
         const bool wrap_top = !single && ze == nz - 1;
@@ -1262,14 +1317,15 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
 }
 
 
-template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED>
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED, bool XT = false>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
+    static_assert(!XT || ALIGNED, "x tiles: the buffer-load build");
     using Tl = T2Tile<T, WX, WY, VXT>;
     __shared__ typename Tl::Buf bufs[3];
     __shared__ double red[Tl::BLOCK / 64];
     const int g = gridDim.x, b = blockIdx.x;
     const int lb = (g & 7) == 0 ? (b & 7) * (g >> 3) + (b >> 3) : b;  // XCD-contiguous order
-    const int nunits = A.nty * A.chunks;
+    const int nunits = A.ntx * A.nty * A.chunks;  // (ntx = 1 unless XT)
     unsigned int epoch = 0;
     __shared__ unsigned long long clk_lds[kClockWords];
     PhaseClock<TIMED> clk{clk_lds};
@@ -1284,12 +1340,14 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
         T *dst = static_cast<T *>((pass & 1) ? A.buf0 : A.buf1);
         double n0 = 0.0, n1 = 0.0;
         for (int u = lb; u < nunits; u += g) {
-            const int zc = u / A.nty;
-            const int ty = u - zc * A.nty;
+            const int zc = u / (A.nty * A.ntx);
+            const int r = u - zc * (A.nty * A.ntx);
+            const int ty = r / A.ntx, tx = r - ty * A.ntx;
             // 16-byte aligned rows: the two-step lookahead through buffer loads (fields below 2 GiB,
             // which the host guarantees for the ALIGNED builds)
             if constexpr (ALIGNED)
-                sweep3d_t2r<T, WX, WY, VXT, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0, n1, bufs);
+                sweep3d_t2r<T, WX, WY, VXT, FULL, PER, XT>(src, dst, A, tx, ty, zc, steps == 1, want0, want1, n0,
+                                                           n1, bufs);
             else
                 sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0,
                                                                n1, bufs);
@@ -2143,7 +2201,17 @@ static KernelFn pick_t2_shape(bool aligned, bool full) {
 }
 // periodic z: built for the default 512-thread shapes only (null otherwise)
 template <typename T, bool TM>
-static KernelFn pick_t2_kernel(bool aligned, bool full, int block, bool periodic) {
+static KernelFn pick_t2_kernel(bool aligned, bool full, int block, bool periodic, bool xt) {
+    // rows wider than one tile: the 512-thread buffer-load builds with x tiles
+    if (xt) {
+        if (!aligned || block != 512) return nullptr;
+        if constexpr (sizeof(T) == 8)
+            return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, true, TM, true>
+                            : (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, false, TM, true>;
+        else
+            return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, true, TM, true>
+                            : (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, false, TM, true>;
+    }
     if constexpr (sizeof(T) == 8) {
         if (block == 512)
             return periodic ? pick_t2_shape<T, 2, 4, 4, TM, true, true>(aligned, full)
@@ -2232,7 +2300,12 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
     // two sweeps per pass: persistent Dirichlet; 3D with one tile across x
     // (3D periodic: 512-thread workgroups only)
-    const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && p->nx <= kT2Width && p->tile_rows <= 0 &&
+    // rows wider than one tile (x tiles): 16-byte aligned rows, fields below 2 GiB (32-bit buffer
+    // offsets), 512-thread workgroups
+    const bool xt = is3d && p->nx > kT2Width;
+    const bool xt_ok = aligned && c->elems * c->elem_size < 0x80000000ull &&
+                       (p->block_threads <= 0 || p->block_threads == 512);
+    const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && (!xt || xt_ok) && p->tile_rows <= 0 &&
                         (p->bc == CFM_BC_DIRICHLET
                              ? (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512)
                              : (p->block_threads <= 0 || p->block_threads == 512));
@@ -2242,14 +2315,20 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     const bool t2_ok = t2_ok3 || t2_ok2;
     if (p->time_block == 2 && !t2_ok)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent variant, the default tile "
-                                     "shape and, in 2D, Dirichlet rows, in 3D nx <= %d (periodic z: 512-thread "
-                                     "workgroups)", kT2Width);
+                                     "shape and, in 2D, Dirichlet rows, in 3D 512-thread workgroups for periodic z "
+                                     "and, with nx > %d, 16-byte aligned rows and a field below 2 GiB", kT2Width);
     if (p->time_block >= 3 && !t2_ok2)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: three or four sweeps per pass need the persistent 2D Dirichlet "
                                      "variant and the default tile shape");
     if (p->time_block == 4 && p->block_threads > 0 && p->block_threads != 512)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: four sweeps per pass run in 512-thread workgroups");
-    c->t2 = t2_ok && p->time_block != 1;
+    // x tiles compute kT2Width columns per tile and own kT2Width - vx of them: auto takes them only
+    // when the tiles' owned share of their columns is at least 0.78 (two sweeps per pass over
+    // one measured 1.45 x that share in FP64 and 1.33 x in FP32: 1022 and 2042 columns 1.47 x and
+    // 1.51 x, 2048 1.09 x, 1536 1.04 x, 1024 0.98 x; FP32 2048 1.05 x, 1024 0.91 x)
+    const int xs = kT2Width - vx;
+    const double xshare = xt ? (double)(p->nx - 2) / ((double)((p->nx - 2 + xs - 1) / xs) * xs) : 1.0;
+    c->t2 = t2_ok && p->time_block != 1 && !(xt && p->time_block == 0 && xshare < 0.78);
     // 2D: four sweeps per pass by default with the default workgroup (16384^2 FP64 976 GLUP/s vs 790
     // for three and 617 for two; FP32 1612 vs 1523 and 1206); three when 1024 threads are asked for
     const int auto2d = (p->block_threads <= 0 || p->block_threads == 512) ? 4 : 3;
@@ -2278,10 +2357,10 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
         const bool al = aligned && c->elems * c->elem_size < 0x80000000ull;
         const bool full = p->nx == kT2Width && al;
         const bool per = p->bc == CFM_BC_PERIODIC;
-        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(al, full, c->block, per)
-                              : pick_t2_kernel<double, false>(al, full, c->block, per))
-                        : (tm ? pick_t2_kernel<float, true>(al, full, c->block, per)
-                              : pick_t2_kernel<float, false>(al, full, c->block, per));
+        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(al, full, c->block, per, xt)
+                              : pick_t2_kernel<double, false>(al, full, c->block, per, xt))
+                        : (tm ? pick_t2_kernel<float, true>(al, full, c->block, per, xt)
+                              : pick_t2_kernel<float, false>(al, full, c->block, per, xt));
     } else {
         c->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
         c->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, c->rows, f64);
@@ -2316,13 +2395,30 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
         tile_h = f64 ? 4 : 8;
     }
     c->ntx = (p->nx + tile_w - 1) / tile_w;
+    // x tiles of the two-sweep 3D kernel: stride kT2Width - vx, each owning the columns after its
+    // first (the edge columns' level 2 needs level 1 one column further out)
+    if (c->t2 && is3d) c->ntx = xt ? (p->nx - 2 + (kT2Width - vx) - 1) / (kT2Width - vx) : 1;
     const int interior = is3d ? p->nz - 2 : p->ny - 2;
     c->nty = is3d ? (p->ny + tile_h - 1) / tile_h : 1;
     const int tiles = c->ntx * c->nty;
     int chunks = p->chunks;
-    if (chunks <= 0) {
+    if (chunks <= 0 && c->t2 && is3d) {
+        // two sweeps per pass in 3D: the chunk count whose units spread best over the grid
+        // (units / (grid x rounds)), each chunk paying two warm-up planes; 512^3 FP64: 128 tiles,
+        // 2 chunks = one unit per workgroup; 1024 x 512 x 256: 384 tiles, 2 chunks = 3 rounds
+        double best = -1.0;
+        for (int ch = 1; ch <= std::max(1, std::min(16, interior / 16)); ++ch) {
+            const long long units = (long long)tiles * ch;
+            const long long rounds = (units + c->grid - 1) / c->grid;
+            const double len = (double)interior / ch;
+            const double eff = (double)units / ((double)rounds * c->grid) * len / (len + 2.0);
+            if (eff > best + 1e-9) {
+                best = eff;
+                chunks = ch;
+            }
+        }
+    } else if (chunks <= 0) {
         // 3D tiles of 2 rows: 4 z-chunks per workgroup balance best; else one
-        // (two sweeps per pass: one unit per workgroup, long chunks)
         const int per_worker = (wg_tile && !c->t2 && c->rows <= 2) ? 4 : 1;
         chunks = std::max(1, per_worker * nworkers / tiles);
         const int min_len = is3d ? 16 : 64;

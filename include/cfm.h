@@ -79,9 +79,10 @@ typedef struct {
     int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
     int block_threads;/* 0 = one workgroup per CU (size set by the tile shape), or 256 */
     int time_block;   /* sweeps per HBM pass of the persistent kernels: 0 = auto (2D Dirichlet: 4,
-                         or 3 with 1024-thread workgroups; 3D: 2 when nx <= 512, Dirichlet or periodic
-                         z (periodic: 512-thread workgroups); with the default tile shape, else 1), 1,
-                         2, or 3 and 4 (2D Dirichlet only; 4 in 512-thread workgroups) */
+                         or 3 with 1024-thread workgroups; 3D: 2, Dirichlet or periodic z (periodic:
+                         512-thread workgroups; nx > 512: x tiles, which need 16-byte aligned rows, a
+                         field below 2 GiB and 512-thread workgroups); with the default tile shape,
+                         else 1), 1, 2, or 3 and 4 (2D Dirichlet only; 4 in 512-thread workgroups) */
     int phase_timers; /* nonzero: every workgroup of the persistent kernel records the time it spends
                          sweeping and waiting at the device-wide barrier (cfm_jacobi_plan_phase_times);
                          results are unchanged */

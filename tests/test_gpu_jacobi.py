@@ -257,6 +257,54 @@ def test_two_sweeps_per_pass_field_of_2gib_and_more(gpu, orc, bc):
     assert t["outer_iterations"] == 2
 
 
+# rows wider than one 512-column tile: x tiles (stride 510 columns in FP64, 508 in FP32, 16-byte
+# aligned) whose edge lanes load the level-0 column beside the tile and recompute the level-1
+# columns the neighbouring tile also computes.  FP64 1022 is two tiles exactly, 1028 three (the
+# last owning 6 columns), 2048 five, 600 two; FP32 1020 three (the last owning 2 columns).
+XT_CASES = [(np.float64, (6, 9, 1022)), (np.float64, (9, 13, 1028)), (np.float64, (5, 7, 2048)),
+            (np.float64, (12, 21, 600)), (np.float32, (6, 9, 1020)), (np.float32, (9, 13, 1028)),
+            (np.float32, (5, 7, 2048)), (np.float32, (12, 21, 600))]
+
+
+@pytest.mark.parametrize("dtype,shape", XT_CASES)
+@pytest.mark.parametrize("bc", [1, 0])
+@pytest.mark.parametrize("niter", [1, 2, 5])
+def test_two_sweeps_per_pass_x_tiles_bitexact(gpu, orc, dtype, shape, bc, niter):
+    """3D rows wider than 512 columns keep two sweeps per pass (x tiles):
+    field and norms equal single sweeps bit for bit, Dirichlet and periodic z."""
+    init = _rand(shape, dtype, 41 + niter)
+    t = _check(gpu, orc, init, niter, bc, norm_every=2, time_block=2)
+    assert t["outer_iterations"] == (niter + 1) // 2
+
+
+@pytest.mark.parametrize("bc", [1, 0])
+@pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (5, 0), (0, 1), (3, 7)])
+def test_two_sweeps_per_pass_x_tiles_units(gpu, orc, bc, chunks, grid):
+    """x tiles across every unit split: one workgroup walking all tiles, chunks of a few planes."""
+    init = _rand((22, 30, 1100), np.float64, 9)
+    t = _check(gpu, orc, init, 9, bc, norm_every=3, time_block=2, chunks=chunks, grid_blocks=grid)
+    assert t["outer_iterations"] == 5
+
+
+@pytest.mark.parametrize("dtype,nx,passes", [(np.float64, 1022, 5), (np.float64, 2048, 5), (np.float64, 1024, 9),
+                                             (np.float64, 1536, 9), (np.float32, 2048, 5), (np.float32, 1024, 9),
+                                             (np.float64, 1023, 9)])
+def test_x_tiles_auto_choice(gpu, orc, dtype, nx, passes):
+    """Auto mode takes x tiles where the tiles own at least 0.78 of the columns they compute
+    (1022, 2048), one sweep per pass otherwise (1024, 1536; 1023: rows not 16-byte aligned)."""
+    init = _rand((6, 9, nx), dtype, 3)
+    t = _check(gpu, orc, init, 9, 1, norm_every=4)
+    assert t["outer_iterations"] == passes
+
+
+def test_two_sweeps_per_pass_x_tiles_large(gpu, orc):
+    """1024 x 256 x 256 FP64 (three x tiles, asked for: auto keeps one sweep per pass at this
+    width), 20 sweeps, periodic z, norms every 5."""
+    init = gpu.init_field(1024, 256, 256)
+    t = _check(gpu, orc, init, 20, 0, norm_every=5, time_block=2)
+    assert t["outer_iterations"] == 10
+
+
 def test_reference_mode_3d_full_size_bitexact(gpu, orc):
     """The jacobi3D CLI's default mode at the benchmark size: 512^3 FP64, sin
     x faces, periodic z halos, 1000 sweeps (two per HBM pass), update norm
@@ -357,7 +405,8 @@ def test_two_sweeps_per_pass_matches_one_sweep_per_pass_2d(gpu):
     assert (ta["outer_iterations"], tb["outer_iterations"]) == (11, 6)
 
 
-@pytest.mark.parametrize("kw", [dict(bc=0, block_threads=1024), dict(bc=1, nx=600), dict(bc=1, variant=0), dict(bc=1, tile_rows=2),
+@pytest.mark.parametrize("kw", [dict(bc=0, block_threads=1024), dict(bc=1, nx=601), dict(bc=1, nx=600, block_threads=1024),
+                                dict(bc=1, variant=0), dict(bc=1, tile_rows=2),
                                 dict(bc=0, nz=1), dict(bc=1, nz=1, tile_rows=8), dict(bc=1, nz=1, variant=0),
                                 dict(bc=1, nz=1, block_threads=256)])
 def test_two_sweeps_per_pass_rejects_unsupported(gpu, kw):

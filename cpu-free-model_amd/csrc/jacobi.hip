@@ -977,7 +977,11 @@ struct Slot {
 // Level 2 is then valid on X0 + 1 .. X0 + TW - 2, and the tile owns (stores and
 // adds to the norms) X0 + 1 .. X0 + TW - E, so consecutive tiles own consecutive
 // columns; each pass recomputes E + 1 level-1 columns per tile boundary.
-template <typename T, int WX, int WY, int VXT, bool FULL, bool PER, bool XT>
+//
+// WIN: read a unit's planes through a window (fields of 2 GiB and more, and the x-tile builds);
+// without it the buffer covers the whole field, which must then be below 2 GiB (the window's
+// bookkeeping costs the 512^3 pass 2 %).
+template <typename T, int WX, int WY, int VXT, bool FULL, bool PER, bool XT, bool WIN>
 __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
                                             int tx, int ty, int zc, bool single, bool want0, bool want1, double &n0,
                                             double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
@@ -1017,9 +1021,25 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         xin[k] = x + k >= 1 && x + k <= nx - 2;
         own[k] = XT ? x + k >= olo && x + k <= ohi : xin[k];
     }
+    // WIN: rows are read through a window of planes zwin .. zwe - 1: the buffer resource starts at
+    // plane zwin and covers the window only, so offsets stay 32-bit for any field size (the host
+    // keeps a chunk's window, chunk_len + 6 planes, below 2 GiB).  Planes outside the window read
+    // zeros.  Without WIN the window is the whole field.
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)src, (short)0, (int)(plane * nz * sizeof(T)), 0x00020000);
-    constexpr unsigned kOut = 0x80000000u;  // beyond every field served here: the load returns zeros
+    __amdgpu_buffer_rsrc_t wrsrc = rsrc;  // WIN: the current window
+    int zwin = 0, zwe = nz;
+    auto window = [&](int z0, int z1) {  // planes z0 .. z1, clamped to the field (wave-uniform)
+        if constexpr (WIN) {
+            z0 = max(z0, 0);
+            z1 = min(z1, nz - 1);
+            zwin = z0;
+            zwe = z1 + 1;
+            wrsrc = __builtin_amdgcn_make_buffer_rsrc((void *)(src + (size_t)z0 * plane), (short)0,
+                                                      (int)((size_t)(zwe - zwin) * plane * sizeof(T)), 0x00020000);
+        }
+    };
+    constexpr unsigned kOut = 0x80000000u;  // beyond every window served here: the load returns zeros
 
     T col[3][2][VX];  // level-0 column rows (own, tile halo) of planes p-1, p, p+1
     T rc[2][2][VX];   // ring: column rows of the two planes in flight
@@ -1033,17 +1053,32 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     // offset carries kOut and the scalar one is 0
     const unsigned vo_in = (unsigned)x * (unsigned)sizeof(T), vo_out = vo_in | kOut;
     auto ldrow = [&](int z, int yy, bool use, T (&v)[VX]) {
-        const bool ok = use && z >= 0 && z < nz && yy >= 0 && yy < ny;  // wave-uniform
-        const unsigned so = ok ? (unsigned)(((unsigned)z * (unsigned)ny + (unsigned)yy) * (unsigned)nx) *
-                                     (unsigned)sizeof(T)
-                               : 0u;
-        const unsigned vo = ok ? vo_in : vo_out;
+        if constexpr (WIN) {
+            const bool ok = use && z >= zwin && z < zwe && yy >= 0 && yy < ny;  // wave-uniform
+            const unsigned so = ok ? (unsigned)(((unsigned)(z - zwin) * (unsigned)ny + (unsigned)yy) * (unsigned)nx) *
+                                         (unsigned)sizeof(T)
+                                   : 0u;
+            const unsigned vo = ok ? vo_in : vo_out;
 #pragma unroll
-        for (int q = 0; q < VX / E; ++q) {
-            const auto t = __builtin_bit_cast(typename Vec16<T>::type,
-                                              __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(vo + 16 * q), (int)so, 0));
+            for (int q = 0; q < VX / E; ++q) {
+                const auto t = __builtin_bit_cast(
+                    typename Vec16<T>::type, __builtin_amdgcn_raw_buffer_load_b128(wrsrc, (int)(vo + 16 * q), (int)so, 0));
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[q * E + e] = t[e];
+                for (int e = 0; e < E; ++e) v[q * E + e] = t[e];
+            }
+        } else {
+            const bool ok = use && z >= 0 && z < nz && yy >= 0 && yy < ny;  // wave-uniform
+            const unsigned so = ok ? (unsigned)(((unsigned)z * (unsigned)ny + (unsigned)yy) * (unsigned)nx) *
+                                         (unsigned)sizeof(T)
+                                   : 0u;
+            const unsigned vo = ok ? vo_in : vo_out;
+#pragma unroll
+            for (int q = 0; q < VX / E; ++q) {
+                const auto t = __builtin_bit_cast(
+                    typename Vec16<T>::type, __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(vo + 16 * q), (int)so, 0));
+#pragma unroll
+                for (int e = 0; e < E; ++e) v[q * E + e] = t[e];
+            }
         }
     };
     auto ld_col = [&](int z, T (&v)[2][VX]) {
@@ -1065,15 +1100,17 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
 #pragma unroll
             for (int sl = 0; sl < 2; ++sl) {
                 const int yy = sl == 0 ? y : yc1;
-                const bool ok = (sl == 0 || has_h) && z >= 0 && z < nz && yy >= 0 && yy < ny;  // wave-uniform
-                const unsigned so = ok ? (unsigned)(((unsigned)z * (unsigned)ny + (unsigned)yy) * (unsigned)nx) *
+                static_assert(!XT || WIN, "x tiles read through plane windows");
+                const bool ok = (sl == 0 || has_h) && z >= zwin && z < zwe && yy >= 0 && yy < ny;  // wave-uniform
+                const unsigned so = ok ? (unsigned)(((unsigned)(z - zwin) * (unsigned)ny + (unsigned)yy) *
+                                                    (unsigned)nx) *
                                              (unsigned)sizeof(T)
                                        : 0u;
                 const unsigned vo = ok ? hvo : kOut;
                 if constexpr (sizeof(T) == 8)
-                    h[sl] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)vo, (int)so, 0));
+                    h[sl] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(wrsrc, (int)vo, (int)so, 0));
                 else
-                    h[sl] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)vo, (int)so, 0));
+                    h[sl] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(wrsrc, (int)vo, (int)so, 0));
             }
         }
     };
@@ -1171,6 +1208,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     const int p1 = single ? ze - 1 : ze;  // inclusive
     if (PER && !single && zb == 1) {
         // level 1 on plane 0 = level 1 on plane nz-2, from stored planes nz-3 .. nz-1
+        window(nz - 3, nz - 1);
         ld_col(nz - 3, col[0]);
         ld_col(nz - 2, col[1]);
         ld_col(nz - 1, col[2]);
@@ -1186,6 +1224,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         }
         p0 = 1;
     }
+    window(p0 - 1, p1 + 3);  // every plane the loop reads (side rows one plane ahead, columns three)
     ld_col(p0 - 1, col[0]);
     ld_col(p0, col[1]);
     ld_col(p0 + 1, col[2]);
@@ -1204,6 +1243,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         typename Tl::Buf &Bp = lds[p % 3];
         bool pin = p >= 1 && p <= nz - 2;
         if constexpr (WRAP) {
+            window(0, 2);
             ld_col(0, col[0]);
             ld_col(1, col[1]);
             ld_col(2, col[2]);
@@ -1317,9 +1357,10 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
 }
 
 
-template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED, bool XT = false>
+template <typename T, int WX, int WY, int VXT, bool ALIGNED, bool FULL, bool PER, bool TIMED, bool XT = false,
+          bool WIN = false>
 __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A) {
-    static_assert(!XT || ALIGNED, "x tiles: the buffer-load build");
+    static_assert(!(XT || WIN) || ALIGNED, "x tiles and plane windows: the buffer-load build");
     using Tl = T2Tile<T, WX, WY, VXT>;
     __shared__ typename Tl::Buf bufs[3];
     __shared__ double red[Tl::BLOCK / 64];
@@ -1346,8 +1387,8 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_t2_kernel(JacobiArgs A)
             // 16-byte aligned rows: the two-step lookahead through buffer loads (fields below 2 GiB,
             // which the host guarantees for the ALIGNED builds)
             if constexpr (ALIGNED)
-                sweep3d_t2r<T, WX, WY, VXT, FULL, PER, XT>(src, dst, A, tx, ty, zc, steps == 1, want0, want1, n0,
-                                                           n1, bufs);
+                sweep3d_t2r<T, WX, WY, VXT, FULL, PER, XT, WIN>(src, dst, A, tx, ty, zc, steps == 1, want0, want1,
+                                                                n0, n1, bufs);
             else
                 sweep3d_t2<T, WX, WY, VXT, ALIGNED, FULL, PER>(src, dst, A, ty, zc, steps == 1, want0, want1, n0,
                                                                n1, bufs);
@@ -2201,16 +2242,24 @@ static KernelFn pick_t2_shape(bool aligned, bool full) {
 }
 // periodic z: built for the default 512-thread shapes only (null otherwise)
 template <typename T, bool TM>
-static KernelFn pick_t2_kernel(bool aligned, bool full, int block, bool periodic, bool xt) {
-    // rows wider than one tile: the 512-thread buffer-load builds with x tiles
-    if (xt) {
+static KernelFn pick_t2_kernel(bool aligned, bool full, int block, bool periodic, bool xt, bool big) {
+    // rows wider than one tile (x tiles) and fields of 2 GiB and more (plane windows): the
+    // 512-thread buffer-load builds
+    if (xt || big) {
         if (!aligned || block != 512) return nullptr;
-        if constexpr (sizeof(T) == 8)
-            return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, true, TM, true>
-                            : (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, false, TM, true>;
-        else
-            return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, true, TM, true>
-                            : (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, false, TM, true>;
+        if constexpr (sizeof(T) == 8) {
+            if (xt)
+                return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, true, TM, true, true>
+                                : (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, false, TM, true, true>;
+            return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, true, TM, false, true>
+                            : (KernelFn)jacobi3d_t2_kernel<T, 2, 4, 4, true, false, false, TM, false, true>;
+        } else {
+            if (xt)
+                return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, true, TM, true, true>
+                                : (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, false, TM, true, true>;
+            return periodic ? (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, true, TM, false, true>
+                            : (KernelFn)jacobi3d_t2_kernel<T, 1, 8, 8, true, false, false, TM, false, true>;
+        }
     }
     if constexpr (sizeof(T) == 8) {
         if (block == 512)
@@ -2300,11 +2349,14 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     // measured defaults on MI355X: 3D FP64 rows 2 x 1024 threads, 3D FP32 rows 4 x 768, 2D rows 4
     // two sweeps per pass: persistent Dirichlet; 3D with one tile across x
     // (3D periodic: 512-thread workgroups only)
-    // rows wider than one tile (x tiles): 16-byte aligned rows, fields below 2 GiB (32-bit buffer
-    // offsets), 512-thread workgroups
+    // the buffer-load (ring) build of the 3D two-sweep kernel reads a unit's planes through a
+    // window of chunk_len + 6 planes with 32-bit offsets: 16-byte aligned rows, planes small enough
+    // for windows of at least 9 (chunk lengths are capped below to keep the window under 2 GiB)
+    const size_t plane_bytes = (size_t)p->nx * p->ny * c->elem_size;
+    const bool ring_ok = aligned && 9 * plane_bytes < 0x80000000ull;
+    // rows wider than one tile (x tiles): the ring build in 512-thread workgroups
     const bool xt = is3d && p->nx > kT2Width;
-    const bool xt_ok = aligned && c->elems * c->elem_size < 0x80000000ull &&
-                       (p->block_threads <= 0 || p->block_threads == 512);
+    const bool xt_ok = ring_ok && (p->block_threads <= 0 || p->block_threads == 512);
     const bool t2_ok3 = is3d && p->variant == CFM_JACOBI_PERSISTENT && (!xt || xt_ok) && p->tile_rows <= 0 &&
                         (p->bc == CFM_BC_DIRICHLET
                              ? (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512)
@@ -2316,19 +2368,19 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     if (p->time_block == 2 && !t2_ok)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent variant, the default tile "
                                      "shape and, in 2D, Dirichlet rows, in 3D 512-thread workgroups for periodic z "
-                                     "and, with nx > %d, 16-byte aligned rows and a field below 2 GiB", kT2Width);
+                                     "and, with nx > %d, 16-byte aligned rows and planes below 2 GiB / 9", kT2Width);
     if (p->time_block >= 3 && !t2_ok2)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: three or four sweeps per pass need the persistent 2D Dirichlet "
                                      "variant and the default tile shape");
     if (p->time_block == 4 && p->block_threads > 0 && p->block_threads != 512)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: four sweeps per pass run in 512-thread workgroups");
     // x tiles compute kT2Width columns per tile and own kT2Width - vx of them: auto takes them only
-    // when the tiles' owned share of their columns is at least 0.78 (two sweeps per pass over
-    // one measured 1.45 x that share in FP64 and 1.33 x in FP32: 1022 and 2042 columns 1.47 x and
-    // 1.51 x, 2048 1.09 x, 1536 1.04 x, 1024 0.98 x; FP32 2048 1.05 x, 1024 0.91 x)
+    // when the tiles' owned share of their columns is at least 0.70 (two sweeps per pass over one,
+    // FP64: 1022 and 2042 columns (share 1.0) 1.47 x and 1.51 x, 2048 (0.80) 1.09 x, 768 (0.75)
+    // 1.32 x, 1536 (0.75) 1.04 x, 1024 (0.67) 0.98 x; FP32 2048 1.05 x, 1536 1.02 x, 1024 0.91 x)
     const int xs = kT2Width - vx;
     const double xshare = xt ? (double)(p->nx - 2) / ((double)((p->nx - 2 + xs - 1) / xs) * xs) : 1.0;
-    c->t2 = t2_ok && p->time_block != 1 && !(xt && p->time_block == 0 && xshare < 0.78);
+    c->t2 = t2_ok && p->time_block != 1 && !(xt && p->time_block == 0 && xshare < 0.70);
     // 2D: four sweeps per pass by default with the default workgroup (16384^2 FP64 976 GLUP/s vs 790
     // for three and 617 for two; FP32 1612 vs 1523 and 1206); three when 1024 threads are asked for
     const int auto2d = (p->block_threads <= 0 || p->block_threads == 512) ? 4 : 3;
@@ -2351,16 +2403,19 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
         // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428
         // GLUP/s (2 columns, 1024 threads); FP32 one wavefront across x (8 columns per lane)
         // 882 vs 844 (4 columns, 1024 threads)
-        // 16-byte aligned rows use buffer loads with 32-bit offsets (the two-step load lookahead,
-        // 512^3 FP64 477 -> 545 GLUP/s): fields of 2 GiB and more take the unaligned build
+        // 16-byte aligned rows use buffer loads with 32-bit offsets into each unit's window of
+        // planes (the two-step load lookahead, 512^3 FP64 477 -> 545 GLUP/s)
         c->block = p->block_threads > 0 ? p->block_threads : 512;
-        const bool al = aligned && c->elems * c->elem_size < 0x80000000ull;
+        // fields of 2 GiB and more: the 512-thread ring builds with plane windows, else the
+        // element-load build (1024 threads)
+        const bool big = c->elems * c->elem_size >= 0x80000000ull;
+        const bool al = ring_ok && (!big || c->block == 512);
         const bool full = p->nx == kT2Width && al;
         const bool per = p->bc == CFM_BC_PERIODIC;
-        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(al, full, c->block, per, xt)
-                              : pick_t2_kernel<double, false>(al, full, c->block, per, xt))
-                        : (tm ? pick_t2_kernel<float, true>(al, full, c->block, per, xt)
-                              : pick_t2_kernel<float, false>(al, full, c->block, per, xt));
+        c->kernel = f64 ? (tm ? pick_t2_kernel<double, true>(al, full && !big, c->block, per, xt, big && al)
+                              : pick_t2_kernel<double, false>(al, full && !big, c->block, per, xt, big && al))
+                        : (tm ? pick_t2_kernel<float, true>(al, full && !big, c->block, per, xt, big && al)
+                              : pick_t2_kernel<float, false>(al, full && !big, c->block, per, xt, big && al));
     } else {
         c->rows = p->tile_rows > 0 ? p->tile_rows : (is3d ? (f64 ? 2 : 4) : 4);
         c->block = p->block_threads > 0 ? p->block_threads : default_block(is3d, c->rows, f64);
@@ -2402,12 +2457,19 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     c->nty = is3d ? (p->ny + tile_h - 1) / tile_h : 1;
     const int tiles = c->ntx * c->nty;
     int chunks = p->chunks;
+    // the ring build's window of chunk_len + 6 planes stays below 2 GiB: the fewest chunks that allows
+    int min_chunks = 1;
+    if (c->t2 && is3d && ring_ok && (xt || c->elems * c->elem_size >= 0x80000000ull)) {
+        const long long max_len = std::max(3LL, (long long)(0x7fffffffull / plane_bytes) - 6);
+        min_chunks = (int)((interior + max_len - 1) / max_len);
+    }
+    if (chunks > 0) chunks = std::max(chunks, min_chunks);
     if (chunks <= 0 && c->t2 && is3d) {
         // two sweeps per pass in 3D: the chunk count whose units spread best over the grid
         // (units / (grid x rounds)), each chunk paying two warm-up planes; 512^3 FP64: 128 tiles,
         // 2 chunks = one unit per workgroup; 1024 x 512 x 256: 384 tiles, 2 chunks = 3 rounds
         double best = -1.0;
-        for (int ch = 1; ch <= std::max(1, std::min(16, interior / 16)); ++ch) {
+        for (int ch = min_chunks; ch <= std::max(min_chunks, std::min(16, interior / 16)); ++ch) {
             const long long units = (long long)tiles * ch;
             const long long rounds = (units + c->grid - 1) / c->grid;
             const double len = (double)interior / ch;

@@ -248,13 +248,24 @@ def test_two_sweeps_per_pass_periodic_units(gpu, orc, norm_every, chunks, grid):
 
 @pytest.mark.parametrize("bc", [1, 0])
 def test_two_sweeps_per_pass_field_of_2gib_and_more(gpu, orc, bc):
-    """Fields of 2 GiB and more take the element-load build (the lookahead
-    build addresses rows with 32-bit buffer offsets): 512 x 512 x 1026 FP64,
-    three sweeps (one two-sweep pass, one single), bit for bit."""
+    """Fields of 2 GiB and more: the buffer-load build reads each unit through
+    a window of planes (32-bit offsets from the window's first plane; the host
+    caps the chunk length so a window stays below 2 GiB): 512 x 512 x 1026
+    FP64, three sweeps (one two-sweep pass, one single), bit for bit."""
     init = _rand((1026, 512, 512), np.float64, 31)
     assert init.nbytes >= 2 ** 31
     t = _check(gpu, orc, init, 3, bc, norm_every=2)
     assert t["outer_iterations"] == 2
+
+
+@pytest.mark.parametrize("bc", [1, 0])
+def test_x_tiles_field_of_2gib_and_more(gpu, orc, bc):
+    """x tiles on a field above 2 GiB: 1022 x 512 x 520 FP64 (two tiles), 5 sweeps, one chunk
+    per tile column forced (chunks=1 is raised to what keeps the windows below 2 GiB)."""
+    init = _rand((520, 512, 1022), np.float64, 32)
+    assert init.nbytes >= 2 ** 31
+    t = _check(gpu, orc, init, 5, bc, norm_every=2, chunks=1)
+    assert t["outer_iterations"] == 3
 
 
 # rows wider than one 512-column tile: x tiles (stride 510 columns in FP64, 508 in FP32, 16-byte
@@ -286,12 +297,13 @@ def test_two_sweeps_per_pass_x_tiles_units(gpu, orc, bc, chunks, grid):
     assert t["outer_iterations"] == 5
 
 
-@pytest.mark.parametrize("dtype,nx,passes", [(np.float64, 1022, 5), (np.float64, 2048, 5), (np.float64, 1024, 9),
-                                             (np.float64, 1536, 9), (np.float32, 2048, 5), (np.float32, 1024, 9),
-                                             (np.float64, 1023, 9)])
+@pytest.mark.parametrize("dtype,nx,passes", [(np.float64, 1022, 5), (np.float64, 2048, 5), (np.float64, 768, 5),
+                                             (np.float64, 1536, 5), (np.float64, 1024, 9), (np.float32, 2048, 5),
+                                             (np.float32, 1024, 9), (np.float64, 1023, 9)])
 def test_x_tiles_auto_choice(gpu, orc, dtype, nx, passes):
-    """Auto mode takes x tiles where the tiles own at least 0.78 of the columns they compute
-    (1022, 2048), one sweep per pass otherwise (1024, 1536; 1023: rows not 16-byte aligned)."""
+    """Auto mode takes x tiles where the tiles own at least 0.70 of the columns they compute
+    (1022, 2048, 768, 1536), one sweep per pass otherwise (1024: 0.67; 1023: rows not 16-byte
+    aligned)."""
     init = _rand((6, 9, nx), dtype, 3)
     t = _check(gpu, orc, init, 9, 1, norm_every=4)
     assert t["outer_iterations"] == passes

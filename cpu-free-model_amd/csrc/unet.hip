@@ -62,7 +62,7 @@ template <int MT, int CIN, int TY, int TZ>
 __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restrict__ X, const bf16 *__restrict__ W,
                                                             int Kp, const float *__restrict__ bias,
                                                             bf16 *__restrict__ Y, int n, int nbx, int nby, int nbz,
-                                                            int relu) {
+                                                            int relu, const bf16 *__restrict__ mask) {
     constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2, NV = CIN / 8;
     constexpr int NT = TY * TZ / 4;
     static_assert(CIN % 8 == 0 && (TY * TZ) % 4 == 0, "shape");
@@ -141,6 +141,12 @@ __global__ void __launch_bounds__(256) conv3_fwd_lds_kernel(const bf16 *__restri
                 if (relu && v < 0.f) v = 0.f;
                 o[i] = f2bf(v);
             }
+            if (mask) {  // ReLU backward: zero where the forward activation was not positive
+                const bf16x4 mk = *reinterpret_cast<const bf16x4 *>(mask + p * Cout + m * 16 + 4 * g);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (!(bf2f(mk[i]) > 0.f)) o[i] = (bf16)0.0f;
+            }
             *reinterpret_cast<bf16x4 *>(Y + p * Cout + m * 16 + 4 * g) = o;
         }
     }
@@ -166,7 +172,7 @@ __global__ void __launch_bounds__(64 * NW) conv3_fwd_xacc_kernel(const bf16 *__r
                                                                  const bf16 *__restrict__ W, int Kp,
                                                                  const float *__restrict__ bias,
                                                                  bf16 *__restrict__ Y, int n, int nby, int nbz,
-                                                                 int relu) {
+                                                                 int relu, const bf16 *__restrict__ mask) {
     constexpr int XW = NXT * 16, HY = TY + 2, HZ = TZ + 2, NV = CIN / 8;
     constexpr int RPW = TY * TZ / NW;
     constexpr int K9 = 9 * CIN, KB = (K9 + 31) / 32;
@@ -277,7 +283,15 @@ __global__ void __launch_bounds__(64 * NW) conv3_fwd_xacc_kernel(const bf16 *__r
                     if (relu && v < 0.f) v = 0.f;
                     o[i] = f2bf(v);
                 }
-                if (out) *reinterpret_cast<bf16x4 *>(Y + p * Cout + m * 16 + 4 * g) = o;
+                if (out) {
+                    if (mask) {  // ReLU backward: zero where the forward activation was not positive
+                        const bf16x4 mk = *reinterpret_cast<const bf16x4 *>(mask + p * Cout + m * 16 + 4 * g);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            if (!(bf2f(mk[i]) > 0.f)) o[i] = (bf16)0.0f;
+                    }
+                    *reinterpret_cast<bf16x4 *>(Y + p * Cout + m * 16 + 4 * g) = o;
+                }
             }
         }
     }
@@ -697,17 +711,6 @@ __global__ void __launch_bounds__(256) chan_sum_kernel(const bf16 *__restrict__ 
 __device__ __forceinline__ bf16x8 ld8(const bf16 *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 __device__ __forceinline__ void st8(bf16 *p, bf16x8 v) { *reinterpret_cast<bf16x8 *>(p) = v; }
 
-__global__ void relu_bwd_kernel(const bf16 *__restrict__ Y, const bf16 *__restrict__ dY, bf16 *__restrict__ dZ,
-                                size_t count) {
-    const size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 8;
-    if (i >= count) return;
-    const bf16x8 y = ld8(Y + i), d = ld8(dY + i);
-    bf16x8 o;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = bf2f(y[j]) > 0.f ? d[j] : (bf16)0.0f;
-    st8(dZ + i, o);
-}
-
 // 2x2x2 average pooling: n -> n/2
 __global__ void pool_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y, int C, int n, int nb) {
     const unsigned m = n / 2, V = C / 8;
@@ -729,7 +732,13 @@ __global__ void pool_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y
     st8(Y + (size_t)q * C + 8 * cv, o);
 }
 
-__global__ void pool_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int n, int nb) {
+// gradient of 2x2x2 average pooling, then (skip != null) plus the skip connection's gradient,
+// channels [skip_off, skip_off + C) of a buffer with skip_ct channels, then (mask != null) the
+// ReLU backward of the layer that produced the pooled tensor: zero where mask ([points][C]) is
+// not positive.  Each step rounds to BF16 as separate kernels would.
+__global__ void pool_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int n, int nb,
+                                const bf16 *__restrict__ skip, int skip_ct, int skip_off,
+                                const bf16 *__restrict__ mask) {
     const unsigned V = C / 8, un = n, m = n / 2;
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (unsigned)nb * un * un * un * V) return;
@@ -739,6 +748,17 @@ __global__ void pool_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ 
     bf16x8 o;
 #pragma unroll
     for (int j = 0; j < 8; ++j) o[j] = f2bf(bf2f(v[j]) * 0.125f);
+    if (skip) {
+        const bf16x8 sk = ld8(skip + (size_t)q * skip_ct + skip_off + 8 * cv);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = f2bf(bf2f(o[j]) + bf2f(sk[j]));
+    }
+    if (mask) {
+        const bf16x8 mk = ld8(mask + (size_t)q * C + 8 * cv);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (!(bf2f(mk[j]) > 0.f)) o[j] = (bf16)0.0f;
+    }
     st8(dX + (size_t)q * C + 8 * cv, o);
 }
 
@@ -753,9 +773,10 @@ __global__ void up_fwd_kernel(const bf16 *__restrict__ X, bf16 *__restrict__ Y, 
     st8(Y + (size_t)q * Ct + off + 8 * cv, ld8(X + ((((size_t)b * um + z / 2) * um + y / 2) * um + x / 2) * C + 8 * cv));
 }
 
-// gradient of nearest upsampling: sum of the 8 children (from channels [off, off+C) of dY)
+// gradient of nearest upsampling: sum of the 8 children (from channels [off, off+C) of dY), then
+// (mask != null) the ReLU backward of the layer that produced the upsampled tensor
 __global__ void up_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX, int C, int Ct, int off, int m,
-                              int nb) {
+                              int nb, const bf16 *__restrict__ mask) {
     const unsigned n = 2 * m, V = C / 8, um = m;
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (unsigned)nb * um * um * um * V) return;
@@ -773,6 +794,12 @@ __global__ void up_bwd_kernel(const bf16 *__restrict__ dY, bf16 *__restrict__ dX
     bf16x8 o;
 #pragma unroll
     for (int j = 0; j < 8; ++j) o[j] = f2bf(a[j]);
+    if (mask) {
+        const bf16x8 mk = ld8(mask + (size_t)q * C + 8 * cv);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (!(bf2f(mk[j]) > 0.f)) o[j] = (bf16)0.0f;
+    }
     st8(dX + (size_t)q * C + 8 * cv, o);
 }
 
@@ -784,17 +811,6 @@ __global__ void chan_copy_kernel(const bf16 *__restrict__ src, int Cs, int offs,
     if (i >= (unsigned)npts * V) return;
     const unsigned p = i / V, cv = i % V;
     st8(dst + (size_t)p * Cd + offd + 8 * cv, ld8(src + (size_t)p * Cs + offs + 8 * cv));
-}
-
-// add two gradient contributions: a += b (elementwise)
-__global__ void add_kernel(bf16 *__restrict__ a, const bf16 *__restrict__ b, size_t count) {
-    const size_t i = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * 8;
-    if (i >= count) return;
-    const bf16x8 x = ld8(a + i), y = ld8(b + i);
-    bf16x8 o;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = f2bf(bf2f(x[j]) + bf2f(y[j]));
-    st8(a + i, o);
 }
 
 // input packing: host-order float planes [b][2][n^3] -> NDHWC BF16 with 8 channels (2 used)
@@ -841,8 +857,9 @@ __global__ void mse_kernel(const float *__restrict__ pred, const float *__restri
     if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
 }
 
-// head backward: dX[p][c] = dpred[p] w[c] (BF16); per-block partial grads of w (16) and b (1),
-// gpart[c][block] (rows for reduce_rows_kernel)
+// head backward: dX[p][c] = dpred[p] w[c] (BF16), zero where X (the ReLU output the head reads)
+// is not positive (the ReLU backward of the last conv); per-block partial grads of w (16) and
+// b (1), gpart[c][block] (rows for reduce_rows_kernel)
 __global__ void head_bwd_kernel(const bf16 *__restrict__ X, const float *__restrict__ w,
                                 const float *__restrict__ dpred, bf16 *__restrict__ dX, float *__restrict__ gpart,
                                 size_t npts) {
@@ -859,7 +876,7 @@ __global__ void head_bwd_kernel(const bf16 *__restrict__ X, const float *__restr
             bf16x8 o;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                o[j] = f2bf(d * w[8 * h + j]);
+                o[j] = bf2f(x[j]) > 0.f ? f2bf(d * w[8 * h + j]) : (bf16)0.0f;
                 gw[8 * h + j] = d * bf2f(x[j]);
             }
             st8(dX + p * 16 + 8 * h, o);
@@ -973,8 +990,8 @@ struct cfm_unet {
     double *lpart = nullptr;
     float *hpart = nullptr;
     // gradients of activations (BF16), sized for the largest of each level
-    bf16 *g1a = nullptr, *g1b = nullptr, *g1c = nullptr, *g1w = nullptr;  // level 1, up to 48 channels
-    bf16 *g2a = nullptr, *g2b = nullptr, *g2c = nullptr, *g2w = nullptr;  // level 2, up to 96 channels
+    bf16 *g1a = nullptr, *g1b = nullptr, *g1c = nullptr;  // level 1, up to 48 channels
+    bf16 *g2a = nullptr, *g2b = nullptr, *g2c = nullptr;  // level 2, up to 96 channels
     bf16 *g3a = nullptr, *g3b = nullptr;                                   // level 3, 64 channels
     long long step = 0;
     std::vector<void *> allocs;
@@ -1069,8 +1086,8 @@ extern "C" cfm_status cfm_unet_create(int n, int batch, cfm_unet **out) {
     A(&u->in_f, 4 * P1 * 2);
     A(&u->lpart, 8 * ((P1 + 255) / 256));
     A(&u->hpart, 4 * 17 * ((P1 + 255) / 256));
-    for (bf16 **g : {&u->g1a, &u->g1b, &u->g1c, &u->g1w}) A(g, 2 * P1 * 48);
-    for (bf16 **g : {&u->g2a, &u->g2b, &u->g2c, &u->g2w}) A(g, 2 * P2 * 96);
+    for (bf16 **g : {&u->g1a, &u->g1b, &u->g1c}) A(g, 2 * P1 * 48);
+    for (bf16 **g : {&u->g2a, &u->g2b, &u->g2c}) A(g, 2 * P2 * 96);
     for (bf16 **g : {&u->g3a, &u->g3b}) A(g, 2 * P3 * 64);
     if (st != CFM_OK) return fail(st);
     if (hipMemset(u->master, 0, 4 * u->nparam) != hipSuccess || hipMemset(u->adam_m, 0, 4 * u->nparam) != hipSuccess ||
@@ -1113,7 +1130,7 @@ extern "C" cfm_status cfm_unet_get_params(cfm_unet *u, float *params) {
 
 template <int MT, int CIN>
 static bool launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
-                            const float *bias, bf16 *Y, int relu) {
+                            const float *bias, bf16 *Y, int relu, const bf16 *mask) {
     if (kp != (27 * CIN + 31) / 32 * 32) return false;
     // tile 16 x TY x TZ points, the staged block at most 80 KiB of LDS (two workgroups per CU): up
     // to 48 input channels 16 x 8 x 2 (four rows per wavefront), 64 channels 16 x 4 x 2, 96
@@ -1124,7 +1141,7 @@ static bool launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, con
     const int nbx = (n + 15) / 16, nby = (n + TY - 1) / TY, nbz = (n + TZ - 1) / TZ;
     const dim3 grid((unsigned)(nbx * nby * nbz * batch));
     hipLaunchKernelGGL((conv3_fwd_lds_kernel<MT, CIN, TY, TZ>), grid, dim3(256), 0, st, X, W, kp, bias, Y, n, nbx, nby,
-                       nbz, relu);
+                       nbz, relu, mask);
     return true;
 }
 
@@ -1138,7 +1155,7 @@ static bool launch_conv_lds(int n, int batch, hipStream_t st, const bf16 *X, con
 // workgroups per CU.
 template <int MT, int CIN, int NXT>
 static void launch_conv_xacc(int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
-                             const float *bias, bf16 *Y, int relu) {
+                             const float *bias, bf16 *Y, int relu, const bf16 *mask) {
     constexpr bool two = MT * NXT <= 4;
     constexpr int lds8 = NXT * 16 * 6 * 4 * CIN * 2;
     constexpr int NW = two && lds8 > 80 * 1024 ? 8 : 4;
@@ -1146,7 +1163,7 @@ static void launch_conv_xacc(int n, int batch, hipStream_t st, const bf16 *X, co
     static_assert(NXT * 16 * (TY + 2) * (TZ + 2) * CIN * 2 <= 144 * 1024, "staged block above 144 KiB");
     const int nby = (n + TY - 1) / TY, nbz = (n + TZ - 1) / TZ;
     hipLaunchKernelGGL((conv3_fwd_xacc_kernel<MT, CIN, TY, TZ, NXT, NW, 2>), dim3((unsigned)(nby * nbz * batch)),
-                       dim3(64 * NW), 0, st, X, W, kp, bias, Y, n, nby, nbz, relu);
+                       dim3(64 * NW), 0, st, X, W, kp, bias, Y, n, nby, nbz, relu, mask);
 }
 
 // The x-accumulator kernel where it measured faster than the per-tap one: the wide inputs of
@@ -1156,12 +1173,12 @@ static void launch_conv_xacc(int n, int batch, hipStream_t st, const bf16 *X, co
 // ms), and the 16 -> 48 and 32 -> 96 data gradients need 3 x 3 and 3 x 6 accumulator tiles per
 // row: one wavefront per SIMD, twice the time.
 static bool conv_xacc(int cin, int mt, int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
-                      const float *bias, bf16 *Y, int relu) {
+                      const float *bias, bf16 *Y, int relu, const bf16 *mask) {
     if (!CFM_UNET_XACC || n > 64 || kp < 27 * cin) return false;
     const int nxt = n <= 16 ? 1 : n <= 32 ? 2 : 4;
 #define CFM_CONV_XACC(C, M, T)                                               \
     if (cin == C && mt == M && nxt == T) {                                   \
-        launch_conv_xacc<M, C, T>(n, batch, st, X, W, kp, bias, Y, relu);    \
+        launch_conv_xacc<M, C, T>(n, batch, st, X, W, kp, bias, Y, relu, mask); \
         return true;                                                         \
     }
     CFM_CONV_XACC(48, 1, 4)
@@ -1175,10 +1192,10 @@ static bool conv_xacc(int cin, int mt, int n, int batch, hipStream_t st, const b
 
 // the (input channels, output channel tiles) pairs of the network's forward and data-gradient convs
 static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf16 *X, const bf16 *W, int kp,
-                     const float *bias, bf16 *Y, int relu) {
+                     const float *bias, bf16 *Y, int relu, const bf16 *mask) {
 #define CFM_CONV_LDS(C, M)                                                  \
     if (cin == C && mt == M) {                                              \
-        return launch_conv_lds<M, C>(n, batch, st, X, W, kp, bias, Y, relu); \
+        return launch_conv_lds<M, C>(n, batch, st, X, W, kp, bias, Y, relu, mask); \
     }
     CFM_CONV_LDS(8, 1)
     CFM_CONV_LDS(16, 1)
@@ -1197,11 +1214,13 @@ static bool conv_lds(int cin, int mt, int n, int batch, hipStream_t st, const bf
 }
 
 // Y = conv(X) for the (input channels, output channels) pairs of the network
+// mask (may be null): the data gradient of a ReLU layer, zero where mask (the layer's forward
+// output, [points][cout]) is not positive
 static cfm_status conv_fwd(cfm_unet *u, const bf16 *X, int cin, const bf16 *W, int kp, const float *bias, bf16 *Y,
-                           int cout, int level, int relu) {
+                           int cout, int level, int relu, const bf16 *mask = nullptr) {
     const int n = u->n >> level;
-    if (cout % 16 != 0 || !(conv_xacc(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu) ||
-                            conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu)))
+    if (cout % 16 != 0 || !(conv_xacc(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu, mask) ||
+                            conv_lds(cin, cout / 16, n, u->batch, u->stream, X, W, kp, bias, Y, relu, mask)))
         return CFM_FAIL(CFM_EINVAL, "unet: no convolution kernel for %d -> %d channels", cin, cout);
     CFM_HIP(hipGetLastError());
     return CFM_OK;
@@ -1363,16 +1382,11 @@ static cfm_status conv_wgrad(cfm_unet *u, int I, const bf16 *X, const bf16 *dZ) 
     return CFM_OK;
 }
 
-// dX = conv_transpose(dZ) for conv I (output channels = cin of the layer)
-static cfm_status conv_dgrad(cfm_unet *u, int I, const bf16 *dZ, bf16 *dX) {
+// dX = conv_transpose(dZ) for conv I (output channels = cin of the layer); with mask (the
+// forward output of the ReLU layer feeding conv I), the gradient before that ReLU
+static cfm_status conv_dgrad(cfm_unet *u, int I, const bf16 *dZ, bf16 *dX, const bf16 *mask = nullptr) {
     const ConvSpec &s = kConv[I];
-    return conv_fwd(u, dZ, s.cout, u->Wt[I], u->Kpt[I], nullptr, dX, s.cin, s.level, 0);
-}
-
-static cfm_status relu_bwd(cfm_unet *u, const bf16 *Y, const bf16 *dY, bf16 *dZ, size_t count) {
-    hipLaunchKernelGGL(relu_bwd_kernel, blocks_for(count / 8), dim3(256), 0, u->stream, Y, dY, dZ, count);
-    CFM_HIP(hipGetLastError());
-    return CFM_OK;
+    return conv_fwd(u, dZ, s.cout, u->Wt[I], u->Kpt[I], nullptr, dX, s.cin, s.level, 0, mask);
 }
 
 static cfm_status run_backward(cfm_unet *u, const float *target_host, double *loss) {
@@ -1382,7 +1396,8 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     const unsigned nb1 = (unsigned)((P1 + 255) / 256);
     hipLaunchKernelGGL(mse_kernel, dim3(nb1), dim3(256), 0, u->stream, u->pred, u->target, u->dpred, u->lpart, P1);
     const float *hw = u->master + u->head_off;
-    hipLaunchKernelGGL(head_bwd_kernel, dim3(nb1), dim3(256), 0, u->stream, u->A9, hw, u->dpred, u->g1a, u->hpart, P1);
+    // dZ9: the head's data gradient through the last ReLU
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(nb1), dim3(256), 0, u->stream, u->A9, hw, u->dpred, u->g1b, u->hpart, P1);
     CFM_HIP(hipGetLastError());
     // head gradient: reduce per-block partials (fixed order)
     hipLaunchKernelGGL(reduce_rows_kernel, dim3(17), dim3(1024), 0, u->stream, u->hpart, (int)nb1,
@@ -1390,47 +1405,41 @@ static cfm_status run_backward(cfm_unet *u, const float *target_host, double *lo
     cfm_status st;
 #define CK(x) \
     if ((st = (x)) != CFM_OK) return st
+    // Every ReLU backward is folded into the kernel producing the gradient it masks (the data-
+    // gradient convs, the upsampling and pooling gradients, the head), and the skip gradients are
+    // read in place from the concatenated buffers' gradients (g1c, g2c), which stay untouched
+    // until the encoder levels use them.
     // decoder level 1
-    CK(relu_bwd(u, u->A9, u->g1a, u->g1b, P1 * 16));   // dZ9
     CK(conv_wgrad(u, 9, u->A8, u->g1b));
-    CK(conv_dgrad(u, 9, u->g1b, u->g1a));               // dA8
-    CK(relu_bwd(u, u->A8, u->g1a, u->g1b, P1 * 16));   // dZ8
-    CK(conv_wgrad(u, 8, u->U1, u->g1b));
-    CK(conv_dgrad(u, 8, u->g1b, u->g1c));               // dU1 [P1][48]
-    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g1c, u->g2a, 32, 48, 0, n / 2, u->batch);  // dA7
-    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g1c, 48, 32, u->g1w, 16, 0, 16, P1);  // dS1 (skip)
+    CK(conv_dgrad(u, 9, u->g1b, u->g1a, u->A8));       // dZ8
+    CK(conv_wgrad(u, 8, u->U1, u->g1a));
+    CK(conv_dgrad(u, 8, u->g1a, u->g1c));               // dU1 [P1][48]
+    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g1c, u->g2b, 32, 48, 0,
+                       n / 2, u->batch, u->A7);          // dZ7
     // decoder level 2
-    CK(relu_bwd(u, u->A7, u->g2a, u->g2b, P2 * 32));   // dZ7
     CK(conv_wgrad(u, 7, u->A6, u->g2b));
-    CK(conv_dgrad(u, 7, u->g2b, u->g2a));               // dA6
-    CK(relu_bwd(u, u->A6, u->g2a, u->g2b, P2 * 32));   // dZ6
-    CK(conv_wgrad(u, 6, u->U2, u->g2b));
-    CK(conv_dgrad(u, 6, u->g2b, u->g2c));               // dU2 [P2][96]
-    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P3 * 64 / 8), dim3(256), 0, u->stream, u->g2c, u->g3a, 64, 96, 0, n / 4, u->batch);  // dA5
-    hipLaunchKernelGGL(chan_copy_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g2c, 96, 64, u->g2w, 32, 0, 32, P2);  // dS2 (skip)
+    CK(conv_dgrad(u, 7, u->g2b, u->g2a, u->A6));       // dZ6
+    CK(conv_wgrad(u, 6, u->U2, u->g2a));
+    CK(conv_dgrad(u, 6, u->g2a, u->g2c));               // dU2 [P2][96]
+    hipLaunchKernelGGL(up_bwd_kernel, blocks_for(P3 * 64 / 8), dim3(256), 0, u->stream, u->g2c, u->g3b, 64, 96, 0,
+                       n / 4, u->batch, u->A5);          // dZ5
     // bottleneck
-    CK(relu_bwd(u, u->A5, u->g3a, u->g3b, P3 * 64));   // dZ5
     CK(conv_wgrad(u, 5, u->A4, u->g3b));
-    CK(conv_dgrad(u, 5, u->g3b, u->g3a));               // dA4
-    CK(relu_bwd(u, u->A4, u->g3a, u->g3b, P3 * 64));   // dZ4
-    CK(conv_wgrad(u, 4, u->Q2, u->g3b));
-    CK(conv_dgrad(u, 4, u->g3b, u->g3a));               // dQ2 [P3][32]
-    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g3a, u->g2a, 32, n / 2, u->batch);
-    hipLaunchKernelGGL(add_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g2a, u->g2w, P2 * 32);  // dS2
+    CK(conv_dgrad(u, 5, u->g3b, u->g3a, u->A4));       // dZ4
+    CK(conv_wgrad(u, 4, u->Q2, u->g3a));
+    CK(conv_dgrad(u, 4, u->g3a, u->g3b));               // dQ2 [P3][32]
+    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P2 * 32 / 8), dim3(256), 0, u->stream, u->g3b, u->g2b, 32, n / 2,
+                       u->batch, u->g2c, 96, 64, u->S2);  // dZ3 = relu'(S2) (pool' dQ2 + dS2 skip)
     // encoder level 2
-    CK(relu_bwd(u, u->S2, u->g2a, u->g2b, P2 * 32));   // dZ3
     CK(conv_wgrad(u, 3, u->A2, u->g2b));
-    CK(conv_dgrad(u, 3, u->g2b, u->g2a));               // dA2
-    CK(relu_bwd(u, u->A2, u->g2a, u->g2b, P2 * 32));   // dZ2
-    CK(conv_wgrad(u, 2, u->Q1, u->g2b));
-    CK(conv_dgrad(u, 2, u->g2b, u->g2a));               // dQ1 [P2][16]
-    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g2a, u->g1a, 16, n, u->batch);
-    hipLaunchKernelGGL(add_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g1a, u->g1w, P1 * 16);  // dS1
+    CK(conv_dgrad(u, 3, u->g2b, u->g2a, u->A2));       // dZ2
+    CK(conv_wgrad(u, 2, u->Q1, u->g2a));
+    CK(conv_dgrad(u, 2, u->g2a, u->g2b));               // dQ1 [P2][16]
+    hipLaunchKernelGGL(pool_bwd_kernel, blocks_for(P1 * 16 / 8), dim3(256), 0, u->stream, u->g2b, u->g1a, 16, n,
+                       u->batch, u->g1c, 48, 32, u->S1);  // dZ1 = relu'(S1) (pool' dQ1 + dS1 skip)
     // encoder level 1
-    CK(relu_bwd(u, u->S1, u->g1a, u->g1b, P1 * 16));   // dZ1
-    CK(conv_wgrad(u, 1, u->A0, u->g1b));
-    CK(conv_dgrad(u, 1, u->g1b, u->g1a));               // dA0
-    CK(relu_bwd(u, u->A0, u->g1a, u->g1b, P1 * 16));   // dZ0
+    CK(conv_wgrad(u, 1, u->A0, u->g1a));
+    CK(conv_dgrad(u, 1, u->g1a, u->g1b, u->A0));       // dZ0
     CK(conv_wgrad(u, 0, u->X0, u->g1b));
 #undef CK
     CFM_HIP(hipGetLastError());

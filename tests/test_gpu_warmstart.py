@@ -15,7 +15,7 @@ import warmstart as ws
 
 pytestmark = pytest.mark.gpu
 
-NSYS = 16
+NSYS = 128  # held-out systems (the benchmark evaluates 256, the training run 1000)
 
 
 @pytest.fixture(scope="module")
@@ -37,7 +37,7 @@ def test_learned_start_needs_fewer_iterations(gpu, held_out):
     try:
         st0, it0, s0, _, _ = bt.solve_plain()
         x_plain = bt.solutions().copy()
-        with gpu.UNet(meta["n"], 4, params=params) as net:
+        with gpu.UNet(meta["n"], 8, params=params) as net:
             _, (st1, it1, s1, h1, _) = bt.solve_learned(net, meta["u_scale"], hist_cap=1)
         x_learned = bt.solutions()
         rejected = bt.plan.x0_rejected()
@@ -46,8 +46,10 @@ def test_learned_start_needs_fewer_iterations(gpu, held_out):
     assert st0 == gpu.OK and st1 == gpu.OK
     assert not rejected.any()
     bb = np.array([s[3] @ s[3] for s in systems])
-    assert np.all(np.sqrt(h1[:, 0] / bb) < 0.5)  # the smoothed guess is far better than zero
+    r0 = np.sqrt(h1[:, 0] / bb)  # ||b - A x0|| / ||b|| of the smoothed guesses
+    assert np.all(r0 <= 1.0) and np.percentile(r0, 95) < 0.5, np.percentile(r0, [50, 95, 100])
     assert it1.mean() <= 0.93 * it0.mean(), (it0.mean(), it1.mean())
+    assert (it1 < it0).mean() >= 0.9  # nearly every system gains, not only the mean
     for i, s in enumerate(systems):
         assert _true_rel_residual(s, x_plain[i]) <= 2e-8
         assert _true_rel_residual(s, x_learned[i]) <= 2e-8

@@ -1593,7 +1593,12 @@ __device__ __forceinline__ void store_row_nt(T *drow, int x, int nx, const T (&v
     }
 }
 
-template <typename T, int VX, bool ALIGNED, int LA = 2>
+// PER: the reference's periodic y (after every sweep row ny-2 is copied to row 0 and row 1 to
+// row ny-1), as in the 3D two-sweep kernel: level 1 on row 0 is level 1 on row ny-2 (the chunk
+// starting at row 1 computes it first, from stored rows ny-3 .. ny-1), level 1 on row ny-1 is
+// level 1 on row 1 (the chunk ending at row ny-2 computes it in its last step, from stored rows
+// 0 .. 2), and whoever stores rows ny-2 and 1 also writes them whole into rows 0 and ny-1.
+template <typename T, int VX, bool ALIGNED, int LA = 2, bool PER = false>
 __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restrict__ dst, const JacobiArgs &A,
                                            int tx, int yc, bool single, bool want0, bool want1, double &n0,
                                            double &n1) {
@@ -1630,45 +1635,73 @@ __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restr
         v2 = (ok && edge_lane && ex2 >= 0 && ex2 < nx) ? pr[ex2] : T(0);
     };
 
-    const int r0 = single ? yb : yb - 1;
+    // level 1 on one row into l[2] / le[2], from the level-0 rows above (u), at (c) and below (d)
+    auto level1 = [&](const T (&u)[VX], const T (&cr)[VX], const T (&d)[VX], T eu, T ec1, T ec2, T ed, bool rin,
+                      bool nrm) {
+        T west = lane_up1(cr[VX - 1]);
+        T east = lane_dn1(cr[0]);
+        if (lane_w) west = ec1;
+        if (lane_e) east = ec1;
+        if (rin) {  // wave-uniform
+#pragma unroll
+            for (int k = 0; k < VX; ++k) {
+                const T E = (k < VX - 1) ? cr[k + 1] : east;
+                const T W = (k > 0) ? cr[k - 1] : west;
+                const T v = quarter * (E + W + d[k] + u[k]);
+                l[2][k] = xin[k] ? v : cr[k];
+                if (nrm && xin[k]) {
+                    const double dd = (double)v - (double)cr[k];
+                    n0 += dd * dd;
+                }
+            }
+            // edge column: lane 0 -> x0-1 (east neighbour x0 = cr[0]); lane 63 -> x0+SEG (west x0+SEG-1)
+            const T E = lane_w ? cr[0] : ec2;
+            const T W = lane_w ? ec2 : cr[VX - 1];
+            const T v = quarter * (E + W + ed + eu);
+            le[2] = ein ? v : ec1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < VX; ++k) l[2][k] = cr[k];
+            le[2] = ec1;
+        }
+    };
+    // PER: a row stored whole (boundary columns too) as the halo row's copy
+    auto store_copy = [&](int row, const T (&v)[VX]) {
+        T *drow = dst + (size_t)row * nx;
+#pragma unroll
+        for (int k = 0; k < VX; ++k)
+            if (x + k < nx) drow[x + k] = v[k];
+    };
+    // PER: level 1 of a wrapped halo row from three stored rows, read into the first three
+    // window slots (before the window is filled, or in the chunk's last step, after which they are
+    // not read); no norm: its row is another chunk's
+    auto level1_from = [&](int ru) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) load(ru + i, a[i], e1[i], e2[i]);
+        level1(a[0], a[1], a[2], e1[0], e1[1], e2[1], e1[2], true, false);
+    };
+
+    int r0 = single ? yb : yb - 1;
     const int r1 = single ? ye - 1 : ye;  // inclusive
+    if (PER && !single && yb == 1) {
+        // level 1 on row 0 = level 1 on row ny-2, from stored rows ny-3 .. ny-1
+        level1_from(ny - 3);
+#pragma unroll
+        for (int k = 0; k < VX; ++k) l[1][k] = l[2][k];
+        le[1] = le[2];
+        r0 = 1;
+    }
 #pragma unroll
     for (int i = 0; i < NW - 1; ++i) load(r0 - 1 + i, a[i], e1[i], e2[i]);
-    for (int r = r0; r <= r1; ++r) {
-        load(r + 1 + LA, a[NW - 1], e1[NW - 1], e2[NW - 1]);  // consumed LA steps later
+    // after level 1 on row r: level 2 on row r-1 (or the single sweep's store), then the slide
+    auto finish = [&](int r) {
         const bool rin = r >= 1 && r <= ny - 2;
-        // ---- level 1 on row r (own columns, and the edge column for lanes 0 / 63)
-        {
-            const T *cr = a[1];
-            T west = lane_up1(cr[VX - 1]);
-            T east = lane_dn1(cr[0]);
-            if (lane_w) west = e1[1];
-            if (lane_e) east = e1[1];
-            if (rin) {  // wave-uniform
-#pragma unroll
-                for (int k = 0; k < VX; ++k) {
-                    const T E = (k < VX - 1) ? cr[k + 1] : east;
-                    const T W = (k > 0) ? cr[k - 1] : west;
-                    const T v = quarter * (E + W + a[2][k] + a[0][k]);
-                    l[2][k] = xin[k] ? v : cr[k];
-                    if (want0 && xin[k] && r >= yb && r < ye) {
-                        const double d = (double)v - (double)cr[k];
-                        n0 += d * d;
-                    }
-                }
-                // edge column: lane 0 -> x0-1 (east neighbour x0 = cr[0]); lane 63 -> x0+SEG (west x0+SEG-1)
-                const T E = lane_w ? cr[0] : e2[1];
-                const T W = lane_w ? e2[1] : cr[VX - 1];
-                const T v = quarter * (E + W + e1[2] + e1[0]);
-                le[2] = ein ? v : e1[1];
-            } else {
-#pragma unroll
-                for (int k = 0; k < VX; ++k) l[2][k] = cr[k];
-                le[2] = e1[1];
-            }
-        }
         if (single) {
             if (rin) store_row_nt<T, VX, ALIGNED>(dst + (size_t)r * nx, x, nx, l[2], xin);
+            if (PER) {
+                if (r == ny - 2) store_copy(0, l[2]);
+                if (r == 1) store_copy(ny - 1, l[2]);
+            }
         } else if (r - 1 >= yb && r - 1 >= 1 && r - 1 <= ny - 2) {
             // ---- level 2 on row r-1
             const T *cr = l[1];
@@ -1689,6 +1722,10 @@ __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restr
                 }
             }
             store_row_nt<T, VX, ALIGNED>(dst + (size_t)(r - 1) * nx, x, nx, out, xin);
+            if (PER) {
+                if (r - 1 == ny - 2) store_copy(0, out);
+                if (r - 1 == 1) store_copy(ny - 1, out);
+            }
         }
         // ---- slide
 #pragma unroll
@@ -1705,10 +1742,24 @@ __device__ __forceinline__ void sweep2d_t2(const T *__restrict__ src, T *__restr
         }
         le[0] = le[1];
         le[1] = le[2];
+    };
+    // PER: the chunk ending at row ny-2 takes its last step (level 1 on row ny-1 = level 1 on row
+    // 1, from stored rows 0 .. 2) after the loop, so the loop body carries none of it
+    const bool wrap = PER && !single && r1 == ny - 1;
+    const int rl = wrap ? r1 - 1 : r1;
+    for (int r = r0; r <= rl; ++r) {
+        load(r + 1 + LA, a[NW - 1], e1[NW - 1], e2[NW - 1]);  // consumed LA steps later
+        // ---- level 1 on row r (own columns, and the edge column for lanes 0 / 63)
+        level1(a[0], a[1], a[2], e1[0], e1[1], e2[1], e1[2], r >= 1 && r <= ny - 2, want0 && r >= yb && r < ye);
+        finish(r);
+    }
+    if (wrap) {
+        level1_from(0);
+        finish(r1);
     }
 }
 
-template <typename T, bool ALIGNED, int BLOCK, bool TIMED>
+template <typename T, bool ALIGNED, int BLOCK, bool TIMED, bool PER = false>
 __global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
     constexpr int VX = t2d_cols<T>(BLOCK);
     __shared__ double lds[BLOCK / 64];
@@ -1731,7 +1782,7 @@ __global__ void __launch_bounds__(BLOCK) jacobi2d_t2_kernel(JacobiArgs A) {
         for (int u = wave_id; u < nunits; u += nwaves) {
             const int yc = u / A.ntx;
             const int tx = u - yc * A.ntx;
-            sweep2d_t2<T, VX, ALIGNED>(src, dst, A, tx, yc, steps == 1, want0, want1, n0, n1);
+            sweep2d_t2<T, VX, ALIGNED, 2, PER>(src, dst, A, tx, yc, steps == 1, want0, want1, n0, n1);
         }
         if (want0) {
             const double s = block_sum_waves<BLOCK / 64>(n0, lds);
@@ -2167,7 +2218,18 @@ static KernelFn pick_t3d_kernel(bool f64, bool aligned, int block) {
 }
 
 template <bool TM>
-static KernelFn pick_t2d_kernel(bool f64, bool aligned, int block) {
+static KernelFn pick_t2d_kernel(bool f64, bool aligned, int block, bool periodic) {
+    if (periodic) {
+        if (block == 512)
+            return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 512, TM, true>
+                                  : (KernelFn)jacobi2d_t2_kernel<double, false, 512, TM, true>)
+                       : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 512, TM, true>
+                                  : (KernelFn)jacobi2d_t2_kernel<float, false, 512, TM, true>);
+        return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 1024, TM, true>
+                              : (KernelFn)jacobi2d_t2_kernel<double, false, 1024, TM, true>)
+                   : (aligned ? (KernelFn)jacobi2d_t2_kernel<float, true, 1024, TM, true>
+                              : (KernelFn)jacobi2d_t2_kernel<float, false, 1024, TM, true>);
+    }
     if (block == 512)
         return f64 ? (aligned ? (KernelFn)jacobi2d_t2_kernel<double, true, 512, TM>
                               : (KernelFn)jacobi2d_t2_kernel<double, false, 512, TM>)
@@ -2361,15 +2423,17 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
                         (p->bc == CFM_BC_DIRICHLET
                              ? (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512)
                              : (p->block_threads <= 0 || p->block_threads == 512));
-    const bool t2_ok2 = !is3d && p->variant == CFM_JACOBI_PERSISTENT && p->bc == CFM_BC_DIRICHLET &&
-                        p->tile_rows <= 0 &&
+    // 2D: two sweeps per pass with Dirichlet or the reference's periodic y rows, three and four
+    // with Dirichlet rows
+    const bool t2_ok2 = !is3d && p->variant == CFM_JACOBI_PERSISTENT && p->tile_rows <= 0 &&
                         (p->block_threads <= 0 || p->block_threads == 1024 || p->block_threads == 512);
+    const bool t34_ok2 = t2_ok2 && p->bc == CFM_BC_DIRICHLET;
     const bool t2_ok = t2_ok3 || t2_ok2;
     if (p->time_block == 2 && !t2_ok)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: two sweeps per pass needs the persistent variant, the default tile "
-                                     "shape and, in 2D, Dirichlet rows, in 3D 512-thread workgroups for periodic z "
-                                     "and, with nx > %d, 16-byte aligned rows and planes below 2 GiB / 9", kT2Width);
-    if (p->time_block >= 3 && !t2_ok2)
+                                     "shape and, in 3D, 512-thread workgroups for periodic z and, with nx > %d, "
+                                     "16-byte aligned rows and planes below 2 GiB / 9", kT2Width);
+    if (p->time_block >= 3 && !t34_ok2)
         return CFM_FAIL(CFM_ENOTSUP, "jacobi: three or four sweeps per pass need the persistent 2D Dirichlet "
                                      "variant and the default tile shape");
     if (p->time_block == 4 && p->block_threads > 0 && p->block_threads != 512)
@@ -2383,11 +2447,13 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
     c->t2 = t2_ok && p->time_block != 1 && !(xt && p->time_block == 0 && xshare < 0.70);
     // 2D: four sweeps per pass by default with the default workgroup (16384^2 FP64 976 GLUP/s vs 790
     // for three and 617 for two; FP32 1612 vs 1523 and 1206); three when 1024 threads are asked for
-    const int auto2d = (p->block_threads <= 0 || p->block_threads == 512) ? 4 : 3;
+    // (periodic y rows: two)
+    const int auto2d = p->bc != CFM_BC_DIRICHLET ? 2 : (p->block_threads <= 0 || p->block_threads == 512) ? 4 : 3;
     c->sweeps_per_pass =
         !c->t2 ? 1 : p->time_block >= 3 ? p->time_block : (p->time_block == 0 && !is3d) ? auto2d : 2;
     if (c->t2 && !is3d) {
         c->rows = 1;
+        const bool per2 = p->bc == CFM_BC_PERIODIC;
         // 16384^2: FP64 617 GLUP/s at 1024 threads (2 columns per lane) vs 534 at 512;
         // FP32 1205 at 1024 threads vs 942 at 512
         const bool wide_ok = !(c->sweeps_per_pass == 3 && f64 && !aligned);  // (that build spills at 1024)
@@ -2396,8 +2462,8 @@ cfm_status cfm::jacobi_configure(const cfm_jacobi_params *p, JacobiConfig *c) {
                                                   : pick_t4d_kernel<false>(f64, aligned))
                     : c->sweeps_per_pass == 3 ? (tm ? pick_t3d_kernel<true>(f64, aligned, c->block)
                                                     : pick_t3d_kernel<false>(f64, aligned, c->block))
-                                              : (tm ? pick_t2d_kernel<true>(f64, aligned, c->block)
-                                                    : pick_t2d_kernel<false>(f64, aligned, c->block));
+                                              : (tm ? pick_t2d_kernel<true>(f64, aligned, c->block, per2)
+                                                    : pick_t2d_kernel<false>(f64, aligned, c->block, per2));
     } else if (c->t2) {
         c->rows = 1;
         // 512-thread workgroups measured best at 512^3: FP64 4 columns per lane 478 vs 428

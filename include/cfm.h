@@ -79,7 +79,7 @@ typedef struct {
     int spin_limit_us;/* device barrier watchdog; 0 = default (10 s) */
     int block_threads;/* 0 = one workgroup per CU (size set by the tile shape), or 256 */
     int time_block;   /* sweeps per HBM pass of the persistent kernels: 0 = auto (2D Dirichlet: 4,
-                         or 3 with 1024-thread workgroups; 3D: 2, Dirichlet or periodic z (periodic:
+                         or 3 with 1024-thread workgroups; 2D periodic y: 2; 3D: 2, Dirichlet or periodic z (periodic:
                          512-thread workgroups; nx > 512: x tiles, which need 16-byte aligned rows, a
                          field below 2 GiB and 512-thread workgroups); with the default tile shape,
                          else 1), 1, 2, or 3 and 4 (2D Dirichlet only; 4 in 512-thread workgroups) */

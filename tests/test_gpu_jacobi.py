@@ -399,13 +399,38 @@ def test_four_sweeps_per_pass_units_2d(gpu, orc, chunks, grid):
 
 
 @pytest.mark.parametrize("kw", [dict(nz=8, time_block=3), dict(nz=8, time_block=4), dict(nz=1, time_block=4,
-                                                                                          block_threads=1024)])
+                                                                                          block_threads=1024),
+                                dict(nz=1, time_block=3, bc=0), dict(nz=1, time_block=4, bc=0)])
 def test_three_and_four_sweeps_per_pass_rejections(gpu, kw):
     nz = kw.pop("nz")
-    p = gpu.jacobi_params(64, 16, nz, niter=4, bc=1, **kw)
+    bc = kw.pop("bc", 1)
+    p = gpu.jacobi_params(64, 16, nz, niter=4, bc=bc, **kw)
     with pytest.raises(gpu.CfmError) as e:
         gpu.JacobiPlan(p)
     assert e.value.status == gpu.ENOTSUP
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("block", [0, 512])
+@pytest.mark.parametrize("shape", T2_SHAPES_2D)
+@pytest.mark.parametrize("niter", [1, 2, 3, 8])
+def test_two_sweeps_per_pass_periodic_bitexact_2d(gpu, orc, dtype, block, shape, niter):
+    """2D with the reference's periodic y rows (row ny-2 copied to row 0 and row 1 to row ny-1
+    after every sweep), two sweeps per HBM pass: level 1 on the halo rows comes from the opposite
+    end of the field; field (halo rows included) and norms equal single sweeps bit for bit."""
+    init = _rand(shape, dtype, 51 + niter)
+    t = _check(gpu, orc, init, niter, 0, norm_every=3, time_block=2, block_threads=block)
+    assert t["outer_iterations"] == (niter + 1) // 2
+
+
+@pytest.mark.parametrize("norm_every", [1, 3])
+@pytest.mark.parametrize("chunks,grid", [(0, 0), (1, 0), (5, 0), (0, 1), (3, 7), (60, 0)])
+def test_two_sweeps_per_pass_periodic_units_2d(gpu, orc, norm_every, chunks, grid):
+    """Periodic rows across every chunking: one chunk holding both ends, chunks of one row, one
+    workgroup walking every unit; auto mode takes two sweeps per pass for periodic rows."""
+    init = _rand((62, 300), np.float64, 7)
+    t = _check(gpu, orc, init, 9, 0, norm_every=norm_every, chunks=chunks, grid_blocks=grid)
+    assert t["outer_iterations"] == 5
 
 
 def test_two_sweeps_per_pass_matches_one_sweep_per_pass_2d(gpu):
@@ -419,7 +444,7 @@ def test_two_sweeps_per_pass_matches_one_sweep_per_pass_2d(gpu):
 
 @pytest.mark.parametrize("kw", [dict(bc=0, block_threads=1024), dict(bc=1, nx=601), dict(bc=1, nx=600, block_threads=1024),
                                 dict(bc=1, variant=0), dict(bc=1, tile_rows=2),
-                                dict(bc=0, nz=1), dict(bc=1, nz=1, tile_rows=8), dict(bc=1, nz=1, variant=0),
+                                dict(bc=0, nz=1, tile_rows=8), dict(bc=1, nz=1, tile_rows=8), dict(bc=1, nz=1, variant=0),
                                 dict(bc=1, nz=1, block_threads=256)])
 def test_two_sweeps_per_pass_rejects_unsupported(gpu, kw):
     nx = kw.pop("nx", 64)

@@ -302,6 +302,24 @@ def unet_profile_run(cfm, n=64, batch=32, reps=5):
     return out
 
 
+def hbm_ceiling_sweep(cfm, kinds=("copy", "read", "write"), grids=(256, 1024, 4096, 16384, 65536),
+                      unrolls=(1, 4), nts=(False, True), nbytes=1 << 30):
+    """HBM ceiling over probe shapes (1 GiB per buffer, best of 5 launches each): streaming copy
+    (read + write bytes), read only and write only; workgroups of 256 lanes, grid-strided 16-byte
+    vectors, 1 or 4 in flight per lane, plain or non-temporal.  Returns every shape's GB/s and
+    the best per kind."""
+    shapes, best = [], {}
+    for kind in kinds:
+        for grid in grids:
+            for unroll in unrolls:
+                for nt in nts:
+                    gbs, _ = cfm.bandwidth_probe_shape(kind, grid, unroll, nt, nbytes, 5)
+                    shapes.append({"kind": kind, "grid": grid, "unroll": unroll, "nt": nt, "gbs": round(gbs, 1)})
+                    if gbs > best.get(kind, {"gbs": 0.0})["gbs"]:
+                        best[kind] = shapes[-1]
+    return {"hbm_ceiling_bytes": nbytes, "best": best, "shapes": shapes}
+
+
 def warmstart_figures(nsys):
     """Learned warm start on held-out 64^3 variable-coefficient systems: one
     batched plan solved from x0 = 0 and from the trained network's guess
@@ -337,7 +355,9 @@ def main():
         return
     if args.copy_probe:
         gbs, ms = cfm.bandwidth_probe(1 << 30, 5)
-        print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1), "best_ms": round(ms, 4)}))
+        print(json.dumps({"copy_probe_bytes": 1 << 30, "copy_gbs": round(gbs, 1), "best_ms": round(ms, 4)}),
+              flush=True)
+        print(json.dumps(hbm_ceiling_sweep(cfm)), flush=True)
         return
     n, niter = args.n, args.niter
     dev = cfm.device_info(0)
@@ -418,6 +438,10 @@ def main():
         line["cg_warmstart"] = warmstart_figures(args.warmstart_systems)
     gbs, _ = cfm.bandwidth_probe(1 << 30, 5)
     line["roofline"]["copy_ceiling_gbs"] = round(gbs, 1)
+    # the best streaming copy over a few probe shapes (the full sweep: --copy-probe)
+    best = hbm_ceiling_sweep(cfm, kinds=("copy",), grids=(1024, 4096, 65536), unrolls=(1,))["best"]["copy"]
+    line["roofline"]["copy_ceiling_best_gbs"] = best["gbs"]
+    line["roofline"]["copy_ceiling_best_shape"] = {k: best[k] for k in ("grid", "unroll", "nt")}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, args.cpu_seconds)
     else:

@@ -30,7 +30,7 @@ CG_PERSISTENT, CG_DISCRETE = 0, 1
 # Every symbol include/cfm.h declares.
 EXPORTS = (
     "cfm_version", "cfm_status_str", "cfm_last_error", "cfm_device_count", "cfm_device_info", "cfm_device_synchronize",
-    "cfm_bandwidth_probe",
+    "cfm_bandwidth_probe", "cfm_bandwidth_probe_shape",
     "cfm_jacobi_plan_create", "cfm_jacobi_plan_load", "cfm_jacobi_plan_run", "cfm_jacobi_plan_download",
     "cfm_jacobi_plan_field", "cfm_jacobi_plan_destroy", "cfm_jacobi_run", "cfm_jacobi_init_field",
     "cfm_cg_plan_create", "cfm_cg_plan_set_rhs", "cfm_cg_plan_set_x0", "cfm_cg_plan_solve",
@@ -122,6 +122,8 @@ def _load():
     L.cfm_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _ip,
                                   ctypes.POINTER(ctypes.c_size_t)]
     L.cfm_bandwidth_probe.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp, _dp]
+    L.cfm_bandwidth_probe_shape.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, _dp, _dp]
     L.cfm_jacobi_plan_create.argtypes = [ctypes.POINTER(JacobiParams), ctypes.POINTER(_vp)]
     L.cfm_jacobi_plan_load.argtypes = [_vp, _vp]
     L.cfm_jacobi_plan_run.argtypes = [_vp, _dp, ctypes.c_int, _ip, ctypes.POINTER(Timing)]
@@ -228,6 +230,16 @@ def bandwidth_probe(nbytes=1 << 30, reps=5):
     """Best copy bandwidth (GB/s, read + write) of a plain streaming copy."""
     gbs, ms = ctypes.c_double(0), ctypes.c_double(0)
     _check(lib.cfm_bandwidth_probe(nbytes, reps, ctypes.byref(gbs), ctypes.byref(ms)), "cfm_bandwidth_probe")
+    return gbs.value, ms.value
+
+
+def bandwidth_probe_shape(kind, grid, unroll=1, nt=False, nbytes=1 << 30, reps=5):
+    """Best bandwidth (GB/s) of one probe shape: kind "copy" (read + write bytes), "read" or
+    "write"; grid workgroups of 256 lanes, unroll 16-byte vectors in flight per lane."""
+    gbs, ms = ctypes.c_double(0), ctypes.c_double(0)
+    k = {"copy": 0, "read": 1, "write": 2}[kind]
+    _check(lib.cfm_bandwidth_probe_shape(nbytes, reps, k, grid, unroll, 1 if nt else 0, ctypes.byref(gbs),
+                                         ctypes.byref(ms)), "cfm_bandwidth_probe_shape")
     return gbs.value, ms.value
 
 

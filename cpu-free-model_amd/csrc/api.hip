@@ -74,7 +74,105 @@ __global__ void __launch_bounds__(256) copy_probe_kernel(const v2f64_t *__restri
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b[i] = a[i];
 }
+
+// Shapes of the ceiling sweep: KIND 0 copy, 1 read only (a sum per workgroup, so the loads are
+// not removed), 2 write only; U 16-byte vectors in flight per lane (grid-strided); NT
+// non-temporal loads and stores.
+template <int KIND, int U, bool NT>
+__global__ void __launch_bounds__(256) shape_probe_kernel(const v2f64_t *__restrict__ a, v2f64_t *__restrict__ b,
+                                                          size_t n, double *__restrict__ sums) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    v2f64_t acc = {0.0, 0.0};
+    const v2f64_t fill = {1.0, 2.0};
+    for (size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i0 < n; i0 += stride * U) {
+        v2f64_t v[U];
+        if constexpr (KIND != 2) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const size_t i = i0 + u * stride;
+                if (i < n) v[u] = NT ? __builtin_nontemporal_load(a + i) : a[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + u * stride;
+            if (i >= n) continue;
+            if constexpr (KIND == 1) {
+                acc += v[u];
+            } else {
+                const v2f64_t w = KIND == 0 ? v[u] : fill;
+                if constexpr (NT)
+                    __builtin_nontemporal_store(w, b + i);
+                else
+                    b[i] = w;
+            }
+        }
+    }
+    if constexpr (KIND == 1) {
+        __shared__ double red[256];
+        red[threadIdx.x] = acc[0] + acc[1];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+    }
+}
+
+using ShapeKernel = void (*)(const v2f64_t *, v2f64_t *, size_t, double *);
+template <int KIND>
+static ShapeKernel shape_kernel(int unroll, bool nt) {
+    if (unroll == 1) return nt ? shape_probe_kernel<KIND, 1, true> : shape_probe_kernel<KIND, 1, false>;
+    if (unroll == 4) return nt ? shape_probe_kernel<KIND, 4, true> : shape_probe_kernel<KIND, 4, false>;
+    return nullptr;
+}
 }  // namespace
+
+extern "C" cfm_status cfm_bandwidth_probe_shape(size_t bytes, int reps, int kind, int grid, int unroll, int nt,
+                                                double *gbs, double *best_ms) {
+    if (bytes < 16 || reps < 1 || !gbs || kind < 0 || kind > 2 || grid < 1 || grid > (1 << 20))
+        return CFM_FAIL(CFM_EINVAL, "bandwidth probe: bytes >= 16, reps >= 1, kind 0..2, grid 1..2^20");
+    ShapeKernel k = kind == 0 ? shape_kernel<0>(unroll, nt != 0)
+                    : kind == 1 ? shape_kernel<1>(unroll, nt != 0)
+                                : shape_kernel<2>(unroll, nt != 0);
+    if (!k) return CFM_FAIL(CFM_EINVAL, "bandwidth probe: unroll 1 or 4");
+    const size_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    double *sums = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    cfm_status st = CFM_OK;
+    float best = 1e30f;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess ||
+        hipMalloc((void **)&sums, sizeof(double) * grid) != hipSuccess) {
+        st = CFM_ENOMEM;
+    } else if (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+               hipEventCreate(&e1) != hipSuccess) {
+        st = CFM_EDEVICE;
+    } else {
+        for (int r = 0; r < reps + 1 && st == CFM_OK; ++r) {
+            const hipError_t r0 = hipEventRecord(e0, nullptr);
+            hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, nullptr, static_cast<const v2f64_t *>(a),
+                               static_cast<v2f64_t *>(b), n, sums);
+            const hipError_t r1 = hipEventRecord(e1, nullptr);
+            float ms = 0.f;
+            if (r0 != hipSuccess || r1 != hipSuccess || hipGetLastError() != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                st = CFM_EDEVICE;
+            else if (r > 0 && ms < best)
+                best = ms;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (sums) (void)hipFree(sums);
+    if (st != CFM_OK) return st;
+    *gbs = (kind == 0 ? 2.0 : 1.0) * (double)(n * 16) / (best * 1e-3) / 1e9;
+    if (best_ms) *best_ms = best;
+    return CFM_OK;
+}
 
 extern "C" cfm_status cfm_bandwidth_probe(size_t bytes, int reps, double *copy_gbs, double *best_ms) {
     if (bytes < 16 || reps < 1 || !copy_gbs) return CFM_EINVAL;

@@ -144,6 +144,12 @@ cfm_status cfm_device_synchronize(void);
  * calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.  A utility
  * with no counterpart in the reference. */
 cfm_status cfm_bandwidth_probe(size_t bytes, int reps, double *copy_gbs, double *best_ms);
+/* One shape of the HBM ceiling sweep over `bytes`: kind 0 copy (gbs counts read + write bytes),
+ * 1 read only, 2 write only; `grid` workgroups of 256 lanes, `unroll` (1 or 4) 16-byte vectors
+ * in flight per lane, nt nonzero for non-temporal loads and stores.  No counterpart in the
+ * reference. */
+cfm_status cfm_bandwidth_probe_shape(size_t bytes, int reps, int kind, int grid, int unroll, int nt,
+                                     double *gbs, double *best_ms);
 
 /* ------------------------------------------------------------- Jacobi --- */
 /* Replaces the allocate/initialise/launch/swap part of

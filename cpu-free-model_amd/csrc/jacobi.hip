@@ -640,9 +640,13 @@ __global__ void __launch_bounds__(64 * WX * WY) jacobi3d_wg_kernel(JacobiArgs A)
 // Workgroup = WX x WY wavefronts; wave (wx, wy) owns columns
 // [wx*SEG, wx*SEG + SEG) of row y = ty*WY + wy and streams planes p of a
 // z-chunk.  At step p it computes level 1 on plane p from level-0 planes
-// p-1, p, p+1, publishes its level-1 row and x-edge values to LDS, and after
-// the step's one __syncthreads computes level 2 on plane p-1 from level-1
-// planes p-2, p-1, p and the y-neighbour rows of the waves above and below.
+// p-1, p, p+1, publishes its level-1 row and x-edge values to LDS, and
+// computes level 2 on plane p-1 from level-1 planes p-2, p-1, p and the
+// y-neighbour rows of the waves above and below, which the other waves
+// published at step p-1: they are read at the start of step p, so their LDS
+// latency overlaps level 1, and the step's one __syncthreads comes last
+// (1024-thread builds, with half the registers per wave, read them after that
+// barrier, placed between the two levels).
 // The tile's top and bottom waves also compute level 1 on the row just
 // outside the tile (redundantly with the neighbouring tile), and every chunk
 // recomputes level 1 on the plane just outside each end.  Every level-1 value
@@ -664,6 +668,9 @@ struct T2Tile {
     static constexpr int SEG = 64 * VX;
     static constexpr int TW = SEG * WX;
     static constexpr int BLOCK = 64 * WX * WY;
+    // level-2 rows read before level 1 (their registers live across it): up to 512 threads, where a
+    // wave has 256 VGPRs; the 1024-thread builds (128 VGPRs) read them after the step's barrier
+    static constexpr bool HOIST = BLOCK <= 512;
     struct Buf {
         T row[WY][TW];         // level-1 own row of each wave (plane p)
         T l1e[WY][WX][2];      // its first (lane 0) and last (lane 63) element
@@ -692,6 +699,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
                                            double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
     using Tl = T2Tile<T, WX, WY, VXT>;
     constexpr int VX = Tl::VX;
+    constexpr bool kHoist = Tl::HOIST;
     static_assert(WY >= 2, "the top and bottom wave must differ");
     static_assert(!FULL || ALIGNED, "full-width rows are 16-byte aligned");
     const int lane = threadIdx.x & 63;
@@ -866,6 +874,22 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
             pin = true;
         }
         const bool nrm = want0 && p >= zb && p < ze;
+        // level-2 inputs (two-sweep passes): the level-1 rows of plane p-1 and their x edges,
+        // published at step p-1 before the barrier that ended it, are read before level 1 of this
+        // step so that their LDS latency overlaps its arithmetic
+        const int zo = p - 1;
+        const bool l2 = !single && zo >= zb && zo >= 1 && zo <= nz - 2 && y >= 1 && y <= ny - 2;  // wave-uniform
+        const typename Tl::Buf &R = lds[(p + 2) % 3];  // written at step p-1
+        T N[VX], S[VX], wn2 = T(0), en2 = T(0);
+        auto l2_reads = [&] {
+            if (l2) {
+                if (!top) ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x], S);
+                if (!bot) ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x], N);
+                if (lane_w) wn2 = wx > 0 ? R.l1e[wy][wx > 0 ? wx - 1 : 0][1] : T(0);
+                if (lane_e) en2 = wx < WX - 1 ? R.l1e[wy][wx < WX - 1 ? wx + 1 : 0][0] : T(0);
+            }
+        };
+        if constexpr (kHoist) l2_reads();
         level1(Bp, pin, nrm);
         typename Tl::Buf &Bn = lds[(p + 1) % 3];
         if (single) {
@@ -883,29 +907,25 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
             if (lane_w) Bp.l1e[wy][wx][0] = l1o[2][0];
             if (lane_e) Bp.l1e[wy][wx][1] = l1o[2][VX - 1];
             put_l0e(Bn, col[2]);
-            __syncthreads();
+            if constexpr (!kHoist) {
+                __syncthreads();
+                l2_reads();
+            }
             // ---- level 2 on plane p-1
-            const int zo = p - 1;
-            if (zo >= zb && zo >= 1 && zo <= nz - 2 && y >= 1 && y <= ny - 2) {  // wave-uniform
-                const typename Tl::Buf &R = lds[(p + 2) % 3];  // written at step p-1
-                T N[VX], S[VX];
+            if (l2) {
                 if (top) {
 #pragma unroll
                     for (int k = 0; k < VX; ++k) S[k] = l1h[0][k];
-                } else {
-                    ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][x], S);
                 }
                 if (bot) {
 #pragma unroll
                     for (int k = 0; k < VX; ++k) N[k] = l1h[0][k];
-                } else {
-                    ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][x], N);
                 }
                 const T *cr = l1o[1];
                 T west = lane_up1(cr[VX - 1]);
                 T east = lane_dn1(cr[0]);
-                if (lane_w) west = wx > 0 ? R.l1e[wy][wx > 0 ? wx - 1 : 0][1] : T(0);
-                if (lane_e) east = wx < WX - 1 ? R.l1e[wy][wx < WX - 1 ? wx + 1 : 0][0] : T(0);
+                if (lane_w) west = wn2;
+                if (lane_e) east = en2;
                 T out[VX];
 #pragma unroll
                 for (int k = 0; k < VX; ++k) {
@@ -929,6 +949,7 @@ __device__ __forceinline__ void sweep3d_t2(const T *__restrict__ src, T *__restr
                 if (zo == nz - 2) store_copy(0, l1o[1]);
                 if (zo == 1) store_copy(nz - 1, l1o[1]);
             }
+            if constexpr (kHoist) __syncthreads();  // this step's rows and edges are published for the next step
         }
         // ---- slide the windows
 #pragma unroll
@@ -987,6 +1008,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
                                             double &n1, typename T2Tile<T, WX, WY, VXT>::Buf *lds) {
     using Tl = T2Tile<T, WX, WY, VXT>;
     constexpr int VX = Tl::VX;
+    constexpr bool kHoist = Tl::HOIST;
     constexpr int E = 16 / sizeof(T);
     static_assert(WY >= 2, "the top and bottom wave must differ");
     static_assert(VX % E == 0, "whole 16-byte vectors");
@@ -1258,6 +1280,22 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
             ld_col(p + 3, rc[S]);  // into the window at the end of step p + 1
         }
         const bool nrm = want0 && p >= zb && p < ze;
+        // level-2 inputs (two-sweep passes): the level-1 rows of plane p - 1, published at step p - 1
+        // before the barrier that ended it, are read before level 1 of this step so that their LDS
+        // latency overlaps its arithmetic
+        const int zo = p - 1;
+        const bool l2 = !single && zo >= zb && zo >= 1 && zo <= nz - 2 && y >= 1 && y <= ny - 2;  // wave-uniform
+        const typename Tl::Buf &R = lds[(p + 2) % 3];
+        T N[VX], Sv[VX], wn2 = T(0), en2 = T(0);
+        auto l2_reads = [&] {
+            if (l2) {
+                if (!top) ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][xl], Sv);
+                if (!bot) ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][xl], N);
+                wn2 = R.row[wy][wx > 0 ? wx * Tl::SEG - 1 : 0];
+                en2 = R.row[wy][wx < WX - 1 ? (wx + 1) * Tl::SEG : 0];
+            }
+        };
+        if constexpr (kHoist) l2_reads();
         level1(Bp, pin, nrm);
         typename Tl::Buf &Bn = lds[(p + 1) % 3];
         if (single) {
@@ -1271,27 +1309,22 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
         } else {
             st_vec<T, VX>(&Bp.row[wy][xl], l1o[2]);  // (its x edges are read from here too)
             put_l0e(Bn, col[2]);
-            __syncthreads();
-            const int zo = p - 1;
-            if (zo >= zb && zo >= 1 && zo <= nz - 2 && y >= 1 && y <= ny - 2) {  // wave-uniform
-                const typename Tl::Buf &R = lds[(p + 2) % 3];
-                T N[VX], Sv[VX];
+            if constexpr (!kHoist) {
+                __syncthreads();
+                l2_reads();
+            }
+            if (l2) {
                 if (top) {
 #pragma unroll
                     for (int k = 0; k < VX; ++k) Sv[k] = l1h[0][k];
-                } else {
-                    ld_vec<T, VX>(&R.row[wy - 1 < 0 ? 0 : wy - 1][xl], Sv);
                 }
                 if (bot) {
 #pragma unroll
                     for (int k = 0; k < VX; ++k) N[k] = l1h[0][k];
-                } else {
-                    ld_vec<T, VX>(&R.row[wy + 1 >= WY ? WY - 1 : wy + 1][xl], N);
                 }
                 const T (&cr)[VX] = l1o[1];
                 T west, east;
-                xnb(cr, R.row[wy][wx > 0 ? wx * Tl::SEG - 1 : 0], R.row[wy][wx < WX - 1 ? (wx + 1) * Tl::SEG : 0],
-                    west, east);
+                xnb(cr, wn2, en2, west, east);
                 T out[VX];
 #pragma unroll
                 for (int k = 0; k < VX; ++k) {
@@ -1313,6 +1346,7 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
                 if (zo == nz - 2) store_copy(0, l1o[1]);
                 if (zo == 1) store_copy(nz - 1, l1o[1]);
             }
+            if constexpr (kHoist) __syncthreads();  // this step's rows and edges are published for the next step
         }
         if constexpr (WRAP) return;  // the chunk's last step
         // ---- slide: the window takes the ring slot issued one step earlier

@@ -1033,6 +1033,11 @@ __device__ __forceinline__ void sweep3d_t2r(const T *__restrict__ src, T *__rest
     const T c = stencil_coeff3d<T>();
     const bool top = wy == 0, bot = wy == WY - 1;
     const bool lane_w = lane == 0, lane_e = lane == 63;
+    // FP32: the tile's top and bottom wavefronts, which compute two level-1 rows per step and set
+    // the pace at every barrier, take the issue slot first (512^3: 1075 -> 1102 GLUP/s, periodic
+    // 904 -> 918; FP64 measured 1.4-2.9 % slower with it)
+    if constexpr (sizeof(T) == 4)
+        if (top || bot) __builtin_amdgcn_s_setprio(1);
     const int yc1 = top ? y - 1 : y + 1;
     const bool has_h = top || bot;
     const int ys0 = top ? y - 2 : y - 1;
